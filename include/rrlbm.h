@@ -1,0 +1,215 @@
+/*
+ * rrlbm.h -- C ABI of the MI355X (gfx950) recursively-regularised
+ * lattice-Boltzmann hot path.
+ *
+ * The reference (RR27V33_code_DNS_LES/) exposes its hot path as CUDA kernels
+ * on global FP64 device arrays, called in a fixed order by main()
+ * (RR27V33_code_DNS_LES/LBMLES.h:58-81, main.cu:31-117).  This header offers
+ *
+ *   - one drop-in entry point per reference kernel, operating on FP64 fields in
+ *     the reference layout that the context owns (populations SoA
+ *     f[ip*N + idx], idx = (iz*LY + iy)*LX + ix, plus rho, ux, uy, uz,
+ *     force_realx/y/z and tau_t), so that calling them in the main.cu order
+ *     reproduces the reference step;
+ *   - the fused path: one kernel per time step doing pull-streaming,
+ *     bounce-back, macrovar, RRBC, macrovar_BN, WALE and the RR collision, on
+ *     shifted populations stored in FP64, FP32 or per-direction-scaled FP16;
+ *   - FP64 diagnostics (diag_flow sums and output_profile plane statistics).
+ *
+ * Conventions: every call returns an rrlbm_status (0 = success) and never
+ * exits the process; rrlbm_last_error() gives the message of the last failure
+ * of a context (or of rrlbm_create when ctx is NULL, per host thread).  Work is
+ * enqueued on the context's own HIP stream; calls that return host data
+ * synchronise that stream.  One context per host thread; no global mutable
+ * state.  Sizes are counts of doubles unless stated.
+ *
+ * Velocity ordering: NPOP = 27 uses the channel solver's order
+ * (RR27V33_code_DNS_LES/initial.cu:67-69); NPOP = 19 uses the Taylor-Green
+ * D3Q19 order (Taylor_Green/D3Q19RR.f90:75-77).  rrlbm_lattice() returns them.
+ */
+#ifndef RRLBM_H
+#define RRLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RRLBM_ABI_VERSION 1
+
+typedef enum rrlbm_status {
+    RRLBM_OK = 0,
+    RRLBM_E_INVALID = 1,     /* bad argument or parameter combination */
+    RRLBM_E_HIP = 2,         /* HIP runtime error (message has the HIP error string) */
+    RRLBM_E_STATE = 3,       /* call out of order (e.g. fused step before pack) */
+    RRLBM_E_NUMERIC = 4,     /* non-finite values detected by a diagnostics call */
+    RRLBM_E_IO = 5,          /* file open / read / write failure */
+    RRLBM_E_UNSUPPORTED = 6  /* valid request this build does not provide */
+} rrlbm_status;
+
+/* storage of the fused path's populations */
+typedef enum rrlbm_storage { RRLBM_STORE_F64 = 0, RRLBM_STORE_F32 = 1, RRLBM_STORE_F16 = 2 } rrlbm_storage;
+
+/* reference-layout FP64 fields owned by the context (RR27V33_code_DNS_LES/globals.cu:67-71) */
+typedef enum rrlbm_field {
+    RRLBM_FIELD_F = 0,      /* d_f, NPOP*N */
+    RRLBM_FIELD_F_TEMP = 1, /* d_f_temp, NPOP*N */
+    RRLBM_FIELD_RHO = 2,    /* d_rho, N */
+    RRLBM_FIELD_UX = 3,
+    RRLBM_FIELD_UY = 4,
+    RRLBM_FIELD_UZ = 5,
+    RRLBM_FIELD_FORCEX = 6, /* d_force_realx, N */
+    RRLBM_FIELD_FORCEY = 7,
+    RRLBM_FIELD_FORCEZ = 8,
+    RRLBM_FIELD_TAUT = 9    /* d_taut, N */
+} rrlbm_field;
+
+/* Run-time counterpart of parameters.h (RR27V33_code_DNS_LES/parameters.h:5-24)
+ * and the physics globals (globals.cu:57-63). */
+typedef struct rrlbm_params {
+    int32_t lx, ly, lz;         /* LX, LY, LZ */
+    int32_t npop;               /* NPOP: 27 (D3Q27) or 19 (D3Q19) */
+    int32_t use_les;            /* USE_LES: 0 DNS, 1 WALE (D3Q27 only) */
+    int32_t les_lagged;         /* fused path with LES: 1 = WALE from the previous step's
+                                   velocity (one launch per step); 0 = reference order
+                                   (a velocity pass, then the step: two launches) */
+    int32_t storage;            /* rrlbm_storage of the fused populations */
+    int32_t wall_x, wall_y, wall_z; /* 1: half-way bounce-back walls on both faces of that
+                                   axis (channel: wall_y); 0: periodic */
+    int32_t wall_rr;            /* 1: RR regularisation of wall-incoming populations (RRBC) */
+    int32_t lid_face;           /* -1 none, else face 2*axis + side (side 0 low, 1 high)
+                                   moving with lid_u (cavity) */
+    double tau;                 /* d_tau = 3 visc + 1/2 */
+    double force[3];            /* uniform body force of the fused path (d_force_in_x, 0, 0) */
+    double cw;                  /* WALE constant (LES.cu:12) */
+    double lid_u[3];            /* lid velocity */
+    double f16_scale;           /* FP16 storage: (f_i - w_i) * f16_scale / w_i is stored;
+                                   0 selects 4096 */
+    int32_t xcd_remap;          /* 1: workgroup order remapped per XCD (default); 0: linear */
+    int32_t reserved[7];
+} rrlbm_params;
+
+typedef struct rrlbm_ctx rrlbm_ctx;
+
+/* Diagnostics (FP64).  The sums follow diag_flow (output.cu:24-35). */
+typedef struct rrlbm_diag {
+    double mass_excess;        /* sum (rho - 1) */
+    double u_sum[3];           /* sum u, v, w (diag_flow's umean*N, vmean*N, wmean*N) */
+    double momentum[3];        /* sum rho u */
+    double kinetic_energy;     /* sum rho |u|^2 / 2 */
+    double nonfinite_cells;    /* cells with a non-finite rho or kinetic energy */
+    double ncells;             /* N */
+} rrlbm_diag;
+
+/* ---------------------------------------------------------------- setup */
+/* Library version string. */
+const char* rrlbm_version(void);
+/* Defaults of parameters.h / globals.cu: 64^3, D3Q27, DNS, channel walls in y,
+ * RRBC on, Re_tau = 180 physics (globals.cu:57-63). */
+int rrlbm_params_default(rrlbm_params* p);
+/* Velocity set of npop (27 or 19): c[3*i+k], w[i], opp[i]. */
+int rrlbm_lattice(int npop, int32_t* c, double* w, int32_t* opp);
+/* Number of visible HIP devices. */
+int rrlbm_device_count(int* n);
+/* Allocates the reference fields and the fused buffers on the current device
+ * (replaces initialize()'s allocation and constant upload, initial.cu:4-178). */
+int rrlbm_create(const rrlbm_params* p, rrlbm_ctx** out);
+/* Frees everything (main.cu:133-150). */
+int rrlbm_destroy(rrlbm_ctx* ctx);
+/* Message of the last failure of ctx (ctx NULL: last rrlbm_create failure of this thread). */
+const char* rrlbm_last_error(const rrlbm_ctx* ctx);
+/* Parameters the context was created with. */
+int rrlbm_get_params(const rrlbm_ctx* ctx, rrlbm_params* out);
+/* Waits for the context's stream. */
+int rrlbm_synchronize(rrlbm_ctx* ctx);
+/* Device bytes held by the context. */
+int rrlbm_memory_bytes(const rrlbm_ctx* ctx, size_t* bytes);
+
+/* ------------------------------------------------ reference-layout fields */
+/* Host <-> device copies of one FP64 field (cudaMemcpy of initial.cu:237-260,
+ * output.cu:19-22,167-171,234).  count must equal the field's size. */
+int rrlbm_upload(rrlbm_ctx* ctx, rrlbm_field field, const double* host, size_t count);
+int rrlbm_download(rrlbm_ctx* ctx, rrlbm_field field, double* host, size_t count);
+
+/* ------------------------------------------- drop-ins for reference kernels */
+/* init_arrays, initial.cu:268-325 (ustar = u0, ystar = visc/u0 of globals.cu:59-61;
+ * the force is params.force[0]). */
+int rrlbm_init_arrays(rrlbm_ctx* ctx, double ustar, double ystar);
+/* forcing, macrovar.cu:63-74 (writes params.force into the force fields). */
+int rrlbm_forcing(rrlbm_ctx* ctx);
+/* forcing_GP, macrovar.cu:76-114 (force_in_x = params.force[0]). */
+int rrlbm_forcing_gp(rrlbm_ctx* ctx, int istep, double rand_phase);
+/* LES_WALE, LES.cu:4-98 (tau_t from the velocity fields). */
+int rrlbm_les_wale(rrlbm_ctx* ctx);
+/* collision_RR, collision.cu:4-119 (in place; uses the force fields and, with
+ * use_les, tau_t). */
+int rrlbm_collision_rr(rrlbm_ctx* ctx);
+/* streaming, streaming.cu:3-24 (push f -> f_temp, periodic). */
+int rrlbm_streaming(rrlbm_ctx* ctx);
+/* BB, streaming.cu:26-55 (half-way bounce-back into f_temp at walled faces). */
+int rrlbm_bb(rrlbm_ctx* ctx);
+/* The D2D copy f_temp -> f of main.cu:64, done as a buffer swap: afterwards
+ * f holds the streamed populations and f_temp the post-collision ones. */
+int rrlbm_swap_f(rrlbm_ctx* ctx);
+/* macrovar, macrovar.cu:4-31. */
+int rrlbm_macrovar(rrlbm_ctx* ctx);
+/* RRBC, BC.cu:4-126 (needs rrlbm_swap_f to have run: re-imposed bounce-back
+ * values at multi-face cells come from f_temp). */
+int rrlbm_rrbc(rrlbm_ctx* ctx);
+/* macrovar_BN, macrovar.cu:33-62. */
+int rrlbm_macrovar_bn(rrlbm_ctx* ctx);
+/* nsteps of main.cu:50-84 with the reference kernels: [LES_WALE] ->
+ * collision_RR -> streaming -> BB -> copy -> macrovar -> [RRBC -> macrovar_BN].
+ * The force fields are used as they are. */
+int rrlbm_step_reference(rrlbm_ctx* ctx, int nsteps);
+
+/* ------------------------------------------------------- initialisation */
+/* Configuration initialisers (no reference counterpart: the configurations'
+ * seeded fields).  They write rho and u and set f to the RR equilibrium. */
+int rrlbm_init_rest(rrlbm_ctx* ctx);
+int rrlbm_init_taylor_green_2d(rrlbm_ctx* ctx, double u0);
+int rrlbm_init_taylor_green_3d(rrlbm_ctx* ctx, double u0);
+/* Channel: u_x = u_tau U+(y+) (viscous / ln law), streaks amp_streak*U(y), noise
+ * amp_noise*U_bulk on every component, seeded counter-based RNG. */
+int rrlbm_init_channel(rrlbm_ctx* ctx, double u_tau, double visc, uint64_t seed, double amp_streak,
+                       double amp_noise);
+/* f = RR equilibrium of the current rho and u fields. */
+int rrlbm_init_equilibrium(rrlbm_ctx* ctx);
+
+/* ------------------------------------------------------------ fused path */
+/* Collide the reference-layout state (the next step's collision_RR, using
+ * params.force and, with use_les, tau_t from LES_WALE) into the fused buffer. */
+int rrlbm_fused_pack(rrlbm_ctx* ctx);
+/* nsteps fused time steps. */
+int rrlbm_fused_step(rrlbm_ctx* ctx, int nsteps);
+/* As rrlbm_fused_step, timing the launches with HIP events on the context's
+ * stream; *ms receives the elapsed milliseconds. */
+int rrlbm_fused_step_timed(rrlbm_ctx* ctx, int nsteps, double* ms);
+/* Finish the current step into the reference fields: streaming, BB, macrovar,
+ * RRBC and macrovar_BN, writing f, rho, u (FP64). */
+int rrlbm_fused_unpack(rrlbm_ctx* ctx);
+/* FP64 diagnostics of the fused state (end-of-step rho, u; no state change). */
+int rrlbm_fused_diag(rrlbm_ctx* ctx, rrlbm_diag* out);
+
+/* ------------------------------------------------------------ diagnostics */
+/* diag_flow sums (output.cu:3-66) over the reference fields. */
+int rrlbm_diag_flow(rrlbm_ctx* ctx, rrlbm_diag* out);
+/* output_profile statistics (output.cu:142-225) over the reference fields:
+ * out[iy*10 + k] = <u>,<v>,<w>,<u'u'>,<v'v'>,<w'w'>,<u'v'>,<u'w'>,<v'w'>,<tau_t>, count = 10*LY. */
+int rrlbm_plane_profiles(rrlbm_ctx* ctx, double* out, size_t count);
+
+/* ---------------------------------------------------------------- files */
+/* output_flow / output_f / restart (output.cu:68-140,227-258, initial.cu:184-265):
+ * data/%09d.dat holds rho, ux, uy, uz (N doubles each); data/f%09d.dat holds
+ * the NPOP*N populations, both raw FP64 in the reference layout. */
+int rrlbm_write_flow(rrlbm_ctx* ctx, const char* dir, int istep);
+int rrlbm_write_f(rrlbm_ctx* ctx, const char* dir, int istep);
+int rrlbm_read_restart(rrlbm_ctx* ctx, const char* dir, int istep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RRLBM_H */

@@ -1,0 +1,350 @@
+// The fused RR-LBM step for gfx950.
+//
+// One launch = one time step of the channel driver's sequence
+// (RR27V33_code_DNS_LES/main.cu:50-84), reordered so that everything after the
+// streaming is cell-local:
+//
+//   pull-stream (streaming.cu:3-24 as a gather) -> half-way bounce-back
+//   (streaming.cu:26-55) -> macrovar (macrovar.cu:4-31) -> RRBC at wall cells
+//   (BC.cu:4-126) -> macrovar_BN (macrovar.cu:33-62) -> [WALE, LES.cu:4-98]
+//   -> RR collision with forcing (collision.cu:4-119) -> store.
+//
+// The stored state between launches is the post-collision populations, kept
+// shifted (f_i - w_i) in FP64, FP32 or per-direction-scaled FP16, SoA with x
+// fastest.  Each of the 27 (19) populations is read once (gathered from the
+// upwind neighbour, periodic wrap in every axis; wall cells then overwrite
+// their incoming populations with the bounce-back value) and written once,
+// coalesced.  Bytes per lattice update: 2*Q*sizeof(storage) (+ velocity field
+// traffic with LES).  One thread per cell, 256-thread workgroups, workgroup
+// order remapped so that each XCD works on a contiguous slab (neighbouring
+// rows share its L2).
+#include "kernels.hpp"
+#include "rr_cell.hpp"
+
+namespace rrlbm {
+
+template <int S>
+RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_t n, int i, uint32_t cell,
+                                                       typename StoreT<S>::C unscale) {
+  using T = typename StoreT<S>::T;
+  using C = typename StoreT<S>::C;
+  const char* p = reinterpret_cast<const char*>(base) + (size_t)i * n * sizeof(T);
+  const uint32_t off = cell * (uint32_t)sizeof(T);
+  const T v = *reinterpret_cast<const T*>(p + off);
+  if constexpr (S == kF16)
+    return C(v) * unscale;
+  else
+    return C(v);
+}
+
+template <int S>
+RRLBM_INLINE __device__ void store_pop(void* base, uint32_t n, int i, uint32_t cell, typename StoreT<S>::C v) {
+  using T = typename StoreT<S>::T;
+  char* p = reinterpret_cast<char*>(base) + (size_t)i * n * sizeof(T);
+  const uint32_t off = cell * (uint32_t)sizeof(T);
+  *reinterpret_cast<T*>(p + off) = T(v);
+}
+
+// Does population i (velocity x,y,z) enter the domain through face f?
+RRLBM_HD constexpr bool enters(int face, int x, int y, int z) {
+  switch (face) {
+    case 0: return x > 0;
+    case 1: return x < 0;
+    case 2: return y > 0;
+    case 3: return y < 0;
+    case 4: return z > 0;
+    default: return z < 0;
+  }
+}
+
+template <class L, typename C, int face>
+RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
+  Moments<C> m;
+  if constexpr (L::Q == 27) m = moments_q27<C>(h);
+  else m = moments_generic<L, C>(h);
+  const Macro<C> q = macro_of<C>(m, Fx, Fy, Fz);
+  C out[L::Q];
+  if constexpr (L::Q == 27) {
+    const NeqTensor<C> N = neq_q27<C>(m, q, Fx, Fy, Fz);
+    rr_post_q27<C, true>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
+  } else {
+    rr_post_generic<L, C>(h, q, Fx, Fy, Fz, C(1), C(-1), out);
+  }
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = out[i];
+  });
+}
+
+// Bounce-back value of population i entering through `face` at this cell:
+// post-collision opposite population of the same cell (+ moving-lid term
+// 6 w_i c_i.u_lid, Cavity/streaming.f90:142-152).
+template <int S, class L>
+RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, int i, int face, uint32_t cell,
+                                                       typename StoreT<S>::C inv_scale) {
+  using C = typename StoreT<S>::C;
+  const int o = L::opp(i);
+  C v = load_pop<S>(a.src, a.n, o, cell, C(L::w(i)) * inv_scale);
+  if (face == a.lid_face)
+    v += C(6.0 * L::w(i)) * (C(L::cx(i)) * C(a.lid_x) + C(L::cy(i)) * C(a.lid_y) + C(L::cz(i)) * C(a.lid_z));
+  return v;
+}
+
+template <int S, class L>
+RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
+                                                         uint32_t x, uint32_t y, uint32_t z, uint32_t cell) {
+  using C = typename StoreT<S>::C;
+  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  const bool on[6] = {(a.walls & 1) && x == 0,        (a.walls & 1) && x == a.lx - 1,
+                      (a.walls & 2) && y == 0,        (a.walls & 2) && y == a.ly - 1,
+                      (a.walls & 4) && z == 0,        (a.walls & 4) && z == a.lz - 1};
+  // half-way bounce-back, faces in the order x-,x+,y-,y+,z-,z+ (last face wins)
+  static_for<6>([&](auto fc) {
+    constexpr int face = decltype(fc)::value;
+    if (!on[face]) return;
+    static_for<L::Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, i, face, cell, inv_scale);
+    });
+  });
+  if (!a.wall_rr) return;
+  // RR regularisation, face by face; later faces first re-impose their own
+  // bounce-back values (Cavity/streaming.f90:91-292).
+  bool done = false;
+  static_for<6>([&](auto fc) {
+    constexpr int face = decltype(fc)::value;
+    if (!on[face]) return;
+    if (done) {
+      static_for<L::Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, i, face, cell, inv_scale);
+      });
+    }
+    regularise_face<L, C, face>(h, Fx, Fy, Fz);
+    done = true;
+  });
+}
+
+template <typename C>
+RRLBM_INLINE __device__ C ld_u(const void* u, uint32_t n, int comp, uint32_t cell) {
+  return reinterpret_cast<const C*>(u)[(size_t)comp * n + cell];
+}
+
+// One-sided wall derivative (LES.cu:19-33): d/dn at the first cell with the
+// half-way wall velocity taken as zero.
+template <typename C>
+RRLBM_INLINE __device__ C ddir(const void* u, uint32_t n, int comp, uint32_t c0, uint32_t cm, uint32_t cp, bool lo,
+                               bool hi) {
+  const C v0 = ld_u<C>(u, n, comp, c0);
+  if (lo) return (C(3) * v0 + ld_u<C>(u, n, comp, cp)) * C(1.0 / 3.0);
+  if (hi) return (C(-3) * v0 - ld_u<C>(u, n, comp, cm)) * C(1.0 / 3.0);
+  return C(0.5) * (ld_u<C>(u, n, comp, cp) - ld_u<C>(u, n, comp, cm));
+}
+
+template <int S, class L, bool WALLS, bool LES, int MODE>
+__global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
+  using C = typename StoreT<S>::C;
+  constexpr int Q = L::Q;
+  uint32_t b = blockIdx.x;
+  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  if (b >= a.nblocks) return;
+  const uint32_t raw = b * kBlock + threadIdx.x;
+  const bool active = raw < a.n;
+  const uint32_t cell = active ? raw : a.n - 1;
+  const uint32_t lxly = a.lx * a.ly;
+  const uint32_t z = fastdiv(cell, a.div_lxly);
+  const uint32_t rem = cell - z * lxly;
+  const uint32_t y = fastdiv(rem, a.div_lx);
+  const uint32_t x = rem - y * a.lx;
+  // upwind source coordinates for c = -1, 0, +1 (periodic wrap)
+  const uint32_t xs[3] = {x + 1 == a.lx ? 0u : x + 1, x, x == 0 ? a.lx - 1 : x - 1};
+  const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
+  const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
+  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+
+  C h[Q];
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+    const uint32_t src = (zs[cz + 1] * a.ly + ys[cy + 1]) * a.lx + xs[cx + 1];
+    h[i] = load_pop<S>(a.src, a.n, i, src, C(L::w(i)) * inv_scale);
+  });
+
+  if constexpr (WALLS) {
+    const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
+                         ((a.walls & 4) && (z == 0 || z == a.lz - 1));
+    if (on_wall) wall_treatment<S, L>(a, h, x, y, z, cell);
+  }
+
+  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  Moments<C> m;
+  if constexpr (Q == 27) m = moments_q27<C>(h);
+  else m = moments_generic<L, C>(h);
+  const Macro<C> q = macro_of<C>(m, Fx, Fy, Fz);
+
+  if constexpr (MODE == kMacro) {
+    if (active) {
+      C* u = reinterpret_cast<C*>(a.u_out);
+      u[cell] = q.ux;
+      u[(size_t)a.n + cell] = q.uy;
+      u[2 * (size_t)a.n + cell] = q.uz;
+    }
+    return;
+  } else if constexpr (MODE == kFinish) {
+    if (active) {
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        a.ref_f[(size_t)i * a.n + cell] = double(h[i]) + L::w(i);
+      });
+      a.ref_rho[cell] = double(q.rho);
+      a.ref_ux[cell] = double(q.ux);
+      a.ref_uy[cell] = double(q.uy);
+      a.ref_uz[cell] = double(q.uz);
+    }
+    return;
+  } else if constexpr (MODE == kDiag) {
+    // FP64 wavefront + workgroup reduction of the end-of-step fields
+    double v[kDiagWidth];
+    const double r = double(q.rho), ux = double(q.ux), uy = double(q.uy), uz = double(q.uz);
+    v[0] = double(q.dr);
+    v[1] = ux; v[2] = uy; v[3] = uz;
+    v[4] = r * ux; v[5] = r * uy; v[6] = r * uz;
+    v[7] = 0.5 * r * (ux * ux + uy * uy + uz * uz);
+    v[8] = (__builtin_isfinite(v[0]) && __builtin_isfinite(v[7])) ? 0.0 : 1.0;
+    __shared__ double red[kBlock / 64][kDiagWidth];
+#pragma unroll
+    for (int k = 0; k < kDiagWidth; ++k) {
+      double t = active ? v[k] : 0.0;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+      v[k] = t;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < kDiagWidth; ++k) red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDiagWidth) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+      a.partials[(size_t)b * kDiagWidth + threadIdx.x] = t;
+    }
+    return;
+  } else {
+    C tau_eff = C(a.tau);
+    if constexpr (LES) {
+      const bool wy = (a.walls & 2) != 0, wx = (a.walls & 1) != 0, wz = (a.walls & 4) != 0;
+      // xs[0] is x+1 and xs[2] is x-1 (the upwind sources for c = -1 and +1)
+      const uint32_t cxp = (z * a.ly + y) * a.lx + xs[0], cxm = (z * a.ly + y) * a.lx + xs[2];
+      const uint32_t cyp = (z * a.ly + ys[0]) * a.lx + x, cym = (z * a.ly + ys[2]) * a.lx + x;
+      const uint32_t czp = (zs[0] * a.ly + y) * a.lx + x, czm = (zs[2] * a.ly + y) * a.lx + x;
+      const bool xlo = wx && x == 0, xhi = wx && x == a.lx - 1;
+      const bool ylo = wy && y == 0, yhi = wy && y == a.ly - 1;
+      const bool zlo = wz && z == 0, zhi = wz && z == a.lz - 1;
+      const void* u = a.u_in;
+      const C dudx = ddir<C>(u, a.n, 0, cell, cxm, cxp, xlo, xhi);
+      const C dvdx = ddir<C>(u, a.n, 1, cell, cxm, cxp, xlo, xhi);
+      const C dwdx = ddir<C>(u, a.n, 2, cell, cxm, cxp, xlo, xhi);
+      const C dudy = ddir<C>(u, a.n, 0, cell, cym, cyp, ylo, yhi);
+      const C dvdy = ddir<C>(u, a.n, 1, cell, cym, cyp, ylo, yhi);
+      const C dwdy = ddir<C>(u, a.n, 2, cell, cym, cyp, ylo, yhi);
+      const C dudz = ddir<C>(u, a.n, 0, cell, czm, czp, zlo, zhi);
+      const C dvdz = ddir<C>(u, a.n, 1, cell, czm, czp, zlo, zhi);
+      const C dwdz = ddir<C>(u, a.n, 2, cell, czm, czp, zlo, zhi);
+      tau_eff += wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
+      if (a.write_u && active) {
+        C* uo = reinterpret_cast<C*>(a.u_out);
+        uo[cell] = q.ux;
+        uo[(size_t)a.n + cell] = q.uy;
+        uo[2 * (size_t)a.n + cell] = q.uz;
+      }
+    }
+    const C omega = C(1) - C(1) / tau_eff;
+    C out[Q];
+    if constexpr (Q == 27) {
+      const NeqTensor<C> N = neq_q27<C>(m, q, Fx, Fy, Fz);
+      if constexpr (S == kF16)
+        rr_post_q27<C, false>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
+      else
+        rr_post_q27<C, true>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+    } else {
+      rr_post_generic<L, C>(h, q, Fx, Fy, Fz, omega, C(1), out);
+      if constexpr (S == kF16) {
+        const C sc = C(a.f16_scale);
+        static_for<Q>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          out[i] = out[i] * (sc * C(1.0 / L::w(i)));
+        });
+      }
+    }
+    if (active) {
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        store_pop<S>(a.dst, a.n, i, cell, out[i]);
+      });
+    }
+  }
+}
+
+// Deterministic final sum of the per-workgroup diagnostics (fixed order).
+__global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* partials, uint32_t nparts, double* out) {
+  __shared__ double red[kBlock];
+  for (int k = 0; k < kDiagWidth; ++k) {
+    double t = 0.0;
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) t += partials[(size_t)p * kDiagWidth + k];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(kBlock), 0, s, partials, nparts, out);
+  return hipGetLastError();
+}
+
+template <int S, class L, bool W, bool LES>
+static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  switch (mode) {
+    case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kFinish: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kFinish>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kDiag: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kDiag>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_storage(int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
+                                 hipStream_t s) {
+  if (q == 27) {
+    if (walls) return les ? launch_mode<S, D3Q27, true, true>(mode, a, grid, s) : launch_mode<S, D3Q27, true, false>(mode, a, grid, s);
+    return les ? launch_mode<S, D3Q27, false, true>(mode, a, grid, s) : launch_mode<S, D3Q27, false, false>(mode, a, grid, s);
+  }
+  if (q == 19) {
+    if (les) return hipErrorInvalidValue;  // WALE is provided on D3Q27 only
+    return walls ? launch_mode<S, D3Q19, true, false>(mode, a, grid, s) : launch_mode<S, D3Q19, false, false>(mode, a, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
+                        hipStream_t s) {
+  switch (storage) {
+    case kF64: return launch_storage<kF64>(q, walls, les, mode, a, grid, s);
+    case kF32: return launch_storage<kF32>(q, walls, les, mode, a, grid, s);
+    case kF16: return launch_storage<kF16>(q, walls, les, mode, a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace rrlbm

@@ -1,0 +1,400 @@
+// Per-cell arithmetic of the recursively regularised (RR) collision.
+//
+// Two formulations of the same operator, both compiled for gfx950 device code:
+//
+//  * rr_post_q27 -- the D3Q27 fast path.  It works in moment space: the 10 raw
+//    moments of the shifted populations come from a 3-stage tensor-product
+//    reduction (D3Q27 = {-1,0,1}^3), the post-collision state is assembled as
+//    the 23 non-zero coefficients of a polynomial in (c_x, c_y, c_z) (degree
+//    <= 2 per axis, which spans every function on the 27 velocities), and the
+//    27 outputs come from a 3-stage tensor-product evaluation of that
+//    polynomial.  About 400 VALU operations per cell instead of the ~3000 of
+//    the per-direction formulation.
+//  * rr_post_generic -- the per-direction formulation of
+//    RR27V33_code_DNS_LES/collision.cu:52-118 with compile-time Hermite
+//    constants.  Used for D3Q19 (Malaspinas third-order pairs,
+//    Taylor_Green/D3Q19RR.f90:131-136,230-235) and as an independent check of
+//    the fast path.
+//
+// The operator (collision.cu:17-118, BC.cu:20-125), with u~ = u/cs, cs^2 = 1/3:
+//   Fbar_i = (c_i.F - u.F)/(2 cs^2)
+//   feq_i  = w_i rho [1 + H1.u~ + (H2:u~u~ + H3:u~u~u~)/2]     (3rd order)
+//   A2     = sum_i H2_i (f_i + Fbar_i feq_i) - rho u~u~
+//   A3     = recursive: A_aab = A_aa u~_b + 2 u~_a A_ab,  A_xyz = A_xy u~_z + A_xz u~_y + A_yz u~_x
+//   fneq_i = w_i (H2_i:A2 + H3_i:A3)/2
+//   collision: f*_i = feq_i + (1 - 1/tau) fneq_i + Fbar_i feq_i
+//   RRBC:      f_i  = feq_i + fneq_i - Fbar_i feq_i
+// Populations are handled shifted, h_i = f_i - w_i (rho0 = 1).
+#pragma once
+#include "lattice.hpp"
+
+namespace rrlbm {
+
+template <typename C>
+struct Moments {
+  C dr;              // sum h_i = rho - 1
+  C jx, jy, jz;      // sum c_i h_i
+  C pxx, pyy, pzz;   // sum c_a c_a h_i
+  C pxy, pxz, pyz;   // sum c_a c_b h_i
+};
+
+// Raw moments of the 27 shifted populations (channel ordering) by three
+// tensor-product stages over x, y, z.
+template <typename C>
+RRLBM_HD RRLBM_INLINE Moments<C> moments_q27(const C (&h)[27]) {
+  // stage x: for each (cy, cz), sums over cx of 1, cx, cx^2
+  C X0[3][3], X1[3][3], X2[3][3];
+  static_for<3>([&](auto jy) {
+    static_for<3>([&](auto jz) {
+      constexpr int cy = decltype(jy)::value - 1, cz = decltype(jz)::value - 1;
+      const C hm = h[D3Q27::index(-1, cy, cz)];
+      const C h0 = h[D3Q27::index(0, cy, cz)];
+      const C hp = h[D3Q27::index(1, cy, cz)];
+      const C s = hp + hm;
+      X0[cy + 1][cz + 1] = s + h0;
+      X1[cy + 1][cz + 1] = hp - hm;
+      X2[cy + 1][cz + 1] = s;
+    });
+  });
+  // stage y: (a,b) in {(0,0),(0,1),(0,2),(1,0),(1,1),(2,0)}
+  C Y00[3], Y01[3], Y02[3], Y10[3], Y11[3], Y20[3];
+  static_for<3>([&](auto kz) {
+    constexpr int z = decltype(kz)::value;
+    const C s0 = X0[2][z] + X0[0][z];
+    Y02[z] = s0;
+    Y00[z] = s0 + X0[1][z];
+    Y01[z] = X0[2][z] - X0[0][z];
+    Y10[z] = (X1[2][z] + X1[0][z]) + X1[1][z];
+    Y11[z] = X1[2][z] - X1[0][z];
+    Y20[z] = (X2[2][z] + X2[0][z]) + X2[1][z];
+  });
+  // stage z
+  Moments<C> m;
+  const C s00 = Y00[2] + Y00[0];
+  m.dr = s00 + Y00[1];
+  m.jz = Y00[2] - Y00[0];
+  m.pzz = s00;
+  m.jx = (Y10[2] + Y10[0]) + Y10[1];
+  m.pxz = Y10[2] - Y10[0];
+  m.jy = (Y01[2] + Y01[0]) + Y01[1];
+  m.pyz = Y01[2] - Y01[0];
+  m.pxx = (Y20[2] + Y20[0]) + Y20[1];
+  m.pyy = (Y02[2] + Y02[0]) + Y02[1];
+  m.pxy = (Y11[2] + Y11[0]) + Y11[1];
+  return m;
+}
+
+// Raw moments for any lattice, direction by direction.
+template <class L, typename C>
+RRLBM_HD RRLBM_INLINE Moments<C> moments_generic(const C (&h)[L::Q]) {
+  Moments<C> m;
+  m.dr = h[0];
+  m.jx = m.jy = m.jz = m.pxx = m.pyy = m.pzz = m.pxy = m.pxz = m.pyz = C(-0.0);
+  static_for<L::Q - 1>([&](auto ic) {
+    constexpr int i = decltype(ic)::value + 1;
+    constexpr int x = L::cx(i), y = L::cy(i), z = L::cz(i);
+    const C v = h[i];
+    m.dr += v;
+    if constexpr (x > 0) m.jx += v; else if constexpr (x < 0) m.jx -= v;
+    if constexpr (y > 0) m.jy += v; else if constexpr (y < 0) m.jy -= v;
+    if constexpr (z > 0) m.jz += v; else if constexpr (z < 0) m.jz -= v;
+    if constexpr (x != 0) m.pxx += v;
+    if constexpr (y != 0) m.pyy += v;
+    if constexpr (z != 0) m.pzz += v;
+    if constexpr (x * y > 0) m.pxy += v; else if constexpr (x * y < 0) m.pxy -= v;
+    if constexpr (x * z > 0) m.pxz += v; else if constexpr (x * z < 0) m.pxz -= v;
+    if constexpr (y * z > 0) m.pyz += v; else if constexpr (y * z < 0) m.pyz -= v;
+  });
+  return m;
+}
+
+template <typename C>
+struct Macro {
+  C rho, dr, ux, uy, uz;
+};
+
+// macrovar (RR27V33_code_DNS_LES/macrovar.cu:17-29): rho = sum f, u = (sum c f + F/2)/rho
+template <typename C>
+RRLBM_HD RRLBM_INLINE Macro<C> macro_of(const Moments<C>& m, C Fx, C Fy, C Fz) {
+  Macro<C> r;
+  r.dr = m.dr;
+  r.rho = C(1) + m.dr;
+  const C inv = C(1) / r.rho;
+  r.ux = (m.jx + C(0.5) * Fx) * inv;
+  r.uy = (m.jy + C(0.5) * Fy) * inv;
+  r.uz = (m.jz + C(0.5) * Fz) * inv;
+  return r;
+}
+
+// Non-equilibrium second moment N = A2/3 in physical units (A2 of
+// collision.cu:72-86), using the D3Q27 closed form of the force projection
+//   sum_i c_a c_b Fbar_i feq_i = S_ab,  S_aa = rho F_a u_a (1 - 3/2 u_a^2),
+//   S_ab = rho (F_a u_b + F_b u_a)/2,
+// which follows from the third lattice moments of the RR equilibrium on D3Q27
+// (sum feq c_a^2 c_b = rho(u_a^2 u_b + u_b/3), sum feq c_x c_y c_z = rho u_x u_y u_z,
+// sum feq c_a^3 = rho u_a).
+template <typename C>
+struct NeqTensor {
+  C xx, yy, zz, xy, xz, yz;
+};
+
+template <typename C>
+RRLBM_HD RRLBM_INLINE NeqTensor<C> neq_q27(const Moments<C>& m, const Macro<C>& q, C Fx, C Fy, C Fz) {
+  const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
+  // sum_i H_aa f_i = 3 P_aa - (sum_i f_i - 1): the populations' own density
+  // excess m.dr (equal to q.dr in the fused step; the drop-in collision takes
+  // rho from the macrovar field, as collision.cu:20 does).
+  const C third = C(1.0 / 3.0) * m.dr;
+  const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
+  NeqTensor<C> n;
+  n.xx = m.pxx + rx * Fx * (C(1) - C(1.5) * ux * ux) - rx * ux - third;
+  n.yy = m.pyy + ry * Fy * (C(1) - C(1.5) * uy * uy) - ry * uy - third;
+  n.zz = m.pzz + rz * Fz * (C(1) - C(1.5) * uz * uz) - rz * uz - third;
+  n.xy = m.pxy + C(0.5) * (rx * Fy + ry * Fx) - rx * uy;
+  n.xz = m.pxz + C(0.5) * (rx * Fz + rz * Fx) - rx * uz;
+  n.yz = m.pyz + C(0.5) * (ry * Fz + rz * Fy) - ry * uz;
+  return n;
+}
+
+// Values of the polynomial k0 + k1 c + k2 c^2 at c = -1, 0, +1.
+template <typename C>
+struct Line3 {
+  C m, z, p;
+};
+template <typename C>
+RRLBM_HD RRLBM_INLINE Line3<C> line3(C k0, C k1, C k2) {
+  const C t = k0 + k2;
+  return {t - k1, k0, t + k1};
+}
+template <typename C>
+RRLBM_HD RRLBM_INLINE Line3<C> line3_nok2(C k0, C k1) {
+  return {k0 - k1, k0, k0 + k1};
+}
+
+// Post-collision (or RRBC) state of one D3Q27 cell.  Writes
+//   out_i = scale_i * (g_i - w_i)/w_i,   g = feq (1 + gs Fbar) + omega fneq,
+// with scale_i = w_i when per_direction_w is true (giving g_i - w_i, the shifted
+// population) and scale_i = scale otherwise (FP16 storage: (g_i - w_i) S / w_i).
+// Collision: omega = 1 - 1/tau_eff, gs = +1.  RRBC: omega = 1, gs = -1.
+template <typename C, bool per_direction_w>
+RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N, C Fx, C Fy, C Fz, C omega,
+                                       C gs, C scale, C (&out)[27]) {
+  const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
+  // feq / w as a polynomial in c: e_abc multiplies c_x^a c_y^b c_z^c
+  const C ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+  const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
+  const C e000m1 = q.dr - C(1.5) * rho * (ux2 + uy2 + uz2);
+  const C e100 = rx * (C(3) - C(4.5) * (uy2 + uz2));
+  const C e010 = ry * (C(3) - C(4.5) * (ux2 + uz2));
+  const C e001 = rz * (C(3) - C(4.5) * (ux2 + uy2));
+  const C e200 = C(4.5) * rx * ux, e020 = C(4.5) * ry * uy, e002 = C(4.5) * rz * uz;
+  const C e110 = C(9) * rx * uy, e101 = C(9) * rx * uz, e011 = C(9) * ry * uz;
+  const C e210 = C(3) * e200 * uy, e201 = C(3) * e200 * uz;
+  const C e120 = C(3) * e020 * ux, e021 = C(3) * e020 * uz;
+  const C e102 = C(3) * e002 * ux, e012 = C(3) * e002 * uy;
+  const C e111 = C(3) * e110 * uz;
+  // fneq / w: same structure with N and the recursive third-order tensor R
+  const C Rxy = N.xx * uy + C(2) * ux * N.xy, Rxz = N.xx * uz + C(2) * ux * N.xz;
+  const C Ryx = N.yy * ux + C(2) * uy * N.xy, Ryz = N.yy * uz + C(2) * uy * N.yz;
+  const C Rzx = N.zz * ux + C(2) * uz * N.xz, Rzy = N.zz * uy + C(2) * uz * N.yz;
+  const C Rxyz = N.xy * uz + N.xz * uy + N.yz * ux;
+  const C wn = omega;
+  const C n000 = C(-1.5) * (N.xx + N.yy + N.zz);
+  // force factor 1 + gs Fbar_i = b0 + phi.c
+  const C k15 = C(1.5) * gs;
+  const C phx = k15 * Fx, phy = k15 * Fy, phz = k15 * Fz;
+  const C bm1 = -(phx * ux + phy * uy + phz * uz);
+  const C b0 = C(1) + bm1;
+  const C e000 = C(1) + e000m1;
+  // coefficients of the product e*(b0 + phi.c) (reduced with c^3 = c) + omega n
+  const C K000 = bm1 + b0 * e000m1 + wn * n000;
+  const C K100 = b0 * e100 + phx * (e000 + e200) + wn * (C(-4.5) * (Ryx + Rzx));
+  const C K010 = b0 * e010 + phy * (e000 + e020) + wn * (C(-4.5) * (Rxy + Rzy));
+  const C K001 = b0 * e001 + phz * (e000 + e002) + wn * (C(-4.5) * (Rxz + Ryz));
+  const C K200 = b0 * e200 + phx * e100 + wn * (C(4.5) * N.xx);
+  const C K020 = b0 * e020 + phy * e010 + wn * (C(4.5) * N.yy);
+  const C K002 = b0 * e002 + phz * e001 + wn * (C(4.5) * N.zz);
+  const C K110 = b0 * e110 + phx * (e010 + e210) + phy * (e100 + e120) + wn * (C(9) * N.xy);
+  const C K101 = b0 * e101 + phx * (e001 + e201) + phz * (e100 + e102) + wn * (C(9) * N.xz);
+  const C K011 = b0 * e011 + phy * (e001 + e021) + phz * (e010 + e012) + wn * (C(9) * N.yz);
+  const C K210 = b0 * e210 + phx * e110 + phy * e200 + wn * (C(13.5) * Rxy);
+  const C K201 = b0 * e201 + phx * e101 + phz * e200 + wn * (C(13.5) * Rxz);
+  const C K120 = b0 * e120 + phx * e020 + phy * e110 + wn * (C(13.5) * Ryx);
+  const C K021 = b0 * e021 + phy * e011 + phz * e020 + wn * (C(13.5) * Ryz);
+  const C K102 = b0 * e102 + phx * e002 + phz * e101 + wn * (C(13.5) * Rzx);
+  const C K012 = b0 * e012 + phy * e002 + phz * e011 + wn * (C(13.5) * Rzy);
+  const C K111 = b0 * e111 + phx * e011 + phy * e101 + phz * e110 + wn * (C(27) * Rxyz);
+  const C K220 = phx * e120 + phy * e210;
+  const C K202 = phx * e102 + phz * e201;
+  const C K022 = phy * e012 + phz * e021;
+  const C K211 = phx * e111 + phy * e201 + phz * e210;
+  const C K121 = phx * e021 + phy * e111 + phz * e120;
+  const C K112 = phx * e012 + phy * e102 + phz * e111;
+  // tensor-product evaluation: x, then y, then z
+  const Line3<C> L00 = line3(K000, K100, K200);
+  const Line3<C> L10 = line3(K010, K110, K210);
+  const Line3<C> L01 = line3(K001, K101, K201);
+  const Line3<C> L20 = line3(K020, K120, K220);
+  const Line3<C> L02 = line3(K002, K102, K202);
+  const Line3<C> L11 = line3(K011, K111, K211);
+  const Line3<C> L21 = line3_nok2(K021, K121);
+  const Line3<C> L12 = line3_nok2(K012, K112);
+  const C L22 = K022;
+  auto pick = [](const Line3<C>& l, int c) -> C { return c < 0 ? l.m : (c == 0 ? l.z : l.p); };
+  static_for<3>([&](auto jx) {
+    constexpr int cx = decltype(jx)::value - 1;
+    const Line3<C> M0 = line3(pick(L00, cx), pick(L10, cx), pick(L20, cx));
+    const Line3<C> M1 = line3(pick(L01, cx), pick(L11, cx), pick(L21, cx));
+    const Line3<C> M2 = line3(pick(L02, cx), pick(L12, cx), L22);
+    static_for<3>([&](auto jy) {
+      constexpr int cy = decltype(jy)::value - 1;
+      const Line3<C> V = line3(pick(M0, cy), pick(M1, cy), pick(M2, cy));
+      static_for<3>([&](auto jz) {
+        constexpr int cz = decltype(jz)::value - 1;
+        constexpr int i = D3Q27::index(cx, cy, cz);
+        const C val = pick(V, cz);
+        if constexpr (per_direction_w)
+          out[i] = val * C(D3Q27::w(i));
+        else
+          out[i] = val * scale;
+      });
+    });
+  });
+}
+
+// Per-direction formulation, following collision.cu:41-118 (D3Q27) and the
+// D3Q19 third-order pairs of D3Q19RR.f90:131-136,221-235,263-281.  h are the
+// shifted populations of the cell; writes out_i = g_i - w_i with
+// g = feq + omega fneq + gs Fbar feq.
+template <class L, typename C>
+RRLBM_HD RRLBM_INLINE C third_order(int /*unused*/, C a210, C a201, C a021, C a120, C a102, C a012, C a111,
+                                    double H210, double H201, double H021, double H120, double H102, double H012,
+                                    double H111) {
+  if constexpr (L::Q == 27) {
+    return C(H210) * a210 + C(H201) * a201 + C(H021) * a021 + C(H120) * a120 + C(H012) * a012 + C(H102) * a102 +
+           C(2.0 * H111) * a111;
+  } else {
+    (void)a111; (void)H111;
+    return C(H210 + H012) * (a210 + a012) + C(H102 + H120) * (a102 + a120) + C(H021 + H201) * (a021 + a201) +
+           C((H210 - H012) / 3.0) * (a210 - a012) + C((H102 - H120) / 3.0) * (a102 - a120) +
+           C((H021 - H201) / 3.0) * (a021 - a201);
+  }
+}
+
+template <class L, typename C>
+RRLBM_HD RRLBM_INLINE void rr_post_generic(const C (&h)[L::Q], const Macro<C>& q, C Fx, C Fy, C Fz, C omega, C gs,
+                                           C (&out)[L::Q]) {
+  constexpr double isq3 = 1.7320508075688772935;  // 1/cs
+  const C rho = q.rho;
+  const C uu = q.ux * C(isq3), vv = q.uy * C(isq3), ww = q.uz * C(isq3);
+  const C a110 = uu * vv, a101 = uu * ww, a011 = vv * ww, a200 = uu * uu, a020 = vv * vv, a002 = ww * ww;
+  const C a210 = a200 * vv, a201 = a200 * ww, a120 = a020 * uu, a021 = a020 * ww, a012 = a002 * vv,
+          a102 = a002 * uu, a111 = a110 * ww;
+  const C G3 = q.ux * Fx + q.uy * Fy + q.uz * Fz;
+  C feqm1[L::Q], G[L::Q];
+  C n110 = C(-0.0), n101 = C(-0.0), n011 = C(-0.0), n200 = C(-0.0), n020 = C(-0.0), n002 = C(-0.0);
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int x = L::cx(i), y = L::cy(i), z = L::cz(i);
+    constexpr double w = L::w(i);
+    constexpr double H100 = Hermite::h1(x), H010 = Hermite::h1(y), H001 = Hermite::h1(z);
+    constexpr double H110 = Hermite::h2(x, y, false), H101 = Hermite::h2(x, z, false), H011 = Hermite::h2(y, z, false);
+    constexpr double H200 = Hermite::h2(x, x, true), H020 = Hermite::h2(y, y, true), H002 = Hermite::h2(z, z, true);
+    constexpr double H210 = Hermite::h3(x, y), H201 = Hermite::h3(x, z), H021 = Hermite::h3(y, z);
+    constexpr double H120 = Hermite::h3(y, x), H102 = Hermite::h3(z, x), H012 = Hermite::h3(z, y);
+    constexpr double H111 = Hermite::hxyz(x, y, z);
+    const C Fbar = C(1.5) * (C(x) * Fx + C(y) * Fy + C(z) * Fz - G3);
+    const C fir = C(H100) * uu + C(H010) * vv + C(H001) * ww;
+    const C sec = C(2.0 * H110) * a110 + C(2.0 * H101) * a101 + C(2.0 * H011) * a011 + C(H200) * a200 +
+                  C(H020) * a020 + C(H002) * a002;
+    const C thir = third_order<L, C>(0, a210, a201, a021, a120, a102, a012, a111, H210, H201, H021, H120, H102,
+                                     H012, H111);
+    const C fm1 = C(w) * (q.dr + rho * (fir + C(0.5) * (sec + thir)));  // feq - w
+    const C g = Fbar * (C(w) + fm1);                                  // Fbar feq
+    feqm1[i] = fm1;
+    G[i] = g;
+    const C proj = h[i] + g;  // (f + Fbar feq) - w; sum_i H2_i w_i = 0
+    if constexpr (H110 != 0.0) n110 += C(H110) * proj;
+    if constexpr (H101 != 0.0) n101 += C(H101) * proj;
+    if constexpr (H011 != 0.0) n011 += C(H011) * proj;
+    n200 += C(H200) * proj;
+    n020 += C(H020) * proj;
+    n002 += C(H002) * proj;
+  });
+  // collision.cu:81-94
+  n110 -= rho * a110; n101 -= rho * a101; n011 -= rho * a011;
+  n200 -= rho * a200; n020 -= rho * a020; n002 -= rho * a002;
+  const C n210 = n200 * vv + C(2) * uu * n110;
+  const C n201 = n200 * ww + C(2) * uu * n101;
+  const C n120 = n020 * uu + C(2) * vv * n110;
+  const C n021 = n020 * ww + C(2) * vv * n011;
+  const C n102 = n002 * uu + C(2) * ww * n101;
+  const C n012 = n002 * vv + C(2) * ww * n011;
+  const C n111 = n110 * ww + n101 * vv + n011 * uu;
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int x = L::cx(i), y = L::cy(i), z = L::cz(i);
+    constexpr double w = L::w(i);
+    constexpr double H110 = Hermite::h2(x, y, false), H101 = Hermite::h2(x, z, false), H011 = Hermite::h2(y, z, false);
+    constexpr double H200 = Hermite::h2(x, x, true), H020 = Hermite::h2(y, y, true), H002 = Hermite::h2(z, z, true);
+    constexpr double H210 = Hermite::h3(x, y), H201 = Hermite::h3(x, z), H021 = Hermite::h3(y, z);
+    constexpr double H120 = Hermite::h3(y, x), H102 = Hermite::h3(z, x), H012 = Hermite::h3(z, y);
+    constexpr double H111 = Hermite::hxyz(x, y, z);
+    const C sec = C(2.0 * H110) * n110 + C(2.0 * H101) * n101 + C(2.0 * H011) * n011 + C(H200) * n200 +
+                  C(H020) * n020 + C(H002) * n002;
+    const C thir = third_order<L, C>(0, n210, n201, n021, n120, n102, n012, n111, H210, H201, H021, H120, H102,
+                                     H012, H111);
+    const C fneq = C(0.5 * w) * (sec + thir);
+    out[i] = feqm1[i] + omega * fneq + gs * G[i];
+  });
+}
+
+// Shifted RR equilibrium of (rho, u): feq_i - w_i (collision.cu:57-69).
+template <class L, typename C>
+RRLBM_HD RRLBM_INLINE void feq_shifted(C rho, C ux, C uy, C uz, C (&out)[L::Q]) {
+  constexpr double isq3 = 1.7320508075688772935;
+  const C uu = ux * C(isq3), vv = uy * C(isq3), ww = uz * C(isq3);
+  const C a110 = uu * vv, a101 = uu * ww, a011 = vv * ww, a200 = uu * uu, a020 = vv * vv, a002 = ww * ww;
+  const C a210 = a200 * vv, a201 = a200 * ww, a120 = a020 * uu, a021 = a020 * ww, a012 = a002 * vv,
+          a102 = a002 * uu, a111 = a110 * ww;
+  const C dr = rho - C(1);
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int x = L::cx(i), y = L::cy(i), z = L::cz(i);
+    constexpr double w = L::w(i);
+    const C fir = C(Hermite::h1(x)) * uu + C(Hermite::h1(y)) * vv + C(Hermite::h1(z)) * ww;
+    const C sec = C(2.0 * Hermite::h2(x, y, false)) * a110 + C(2.0 * Hermite::h2(x, z, false)) * a101 +
+                  C(2.0 * Hermite::h2(y, z, false)) * a011 + C(Hermite::h2(x, x, true)) * a200 +
+                  C(Hermite::h2(y, y, true)) * a020 + C(Hermite::h2(z, z, true)) * a002;
+    const C thir = third_order<L, C>(0, a210, a201, a021, a120, a102, a012, a111, Hermite::h3(x, y),
+                                     Hermite::h3(x, z), Hermite::h3(y, z), Hermite::h3(y, x), Hermite::h3(z, x),
+                                     Hermite::h3(z, y), Hermite::hxyz(x, y, z));
+    out[i] = C(w) * (dr + rho * (fir + C(0.5) * (sec + thir)));
+  });
+}
+
+// WALE eddy relaxation time tau_t = nu_t/cs^2 from the velocity-gradient
+// tensor (RR27V33_code_DNS_LES/LES.cu:79-97).  The fractional powers use
+// x^1.5 = x sqrt(x), x^2.5 = x^2 sqrt(x), x^1.25 = x sqrt(sqrt(x)); a vanishing
+// gradient (0/0 in the reference) gives tau_t = 0.
+template <typename C>
+RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz, C dwdx, C dwdy, C dwdz, C cw) {
+  C sijsq = (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
+  sijsq = C(0.5) * sijsq + dudx * dudx + dvdy * dvdy + dwdz * dwdz;
+  const C s11d = (C(2) * dudx * dudx - dvdy * dvdy - dwdz * dwdz + dudy * dvdx + dudz * dwdx - C(2) * dvdz * dwdy) *
+                 C(1.0 / 3.0);
+  const C s12d = C(0.5) * (dudx * (dudy + dvdx) + dudy * dvdy + dvdx * dvdy + dvdz * dwdx + dudz * dwdy);
+  const C s13d = C(0.5) * (dudx * (dudz + dwdx) + dudy * dvdz + dvdx * dwdy + dudz * dwdz + dwdx * dwdz);
+  const C s22d = (C(2) * dvdy * dvdy - dudx * dudx - dwdz * dwdz + dudy * dvdx + dvdz * dwdy - C(2) * dudz * dwdx) *
+                 C(1.0 / 3.0);
+  const C s23d = C(0.5) * (dvdy * (dvdz + dwdy) + dudz * dvdx + dudy * dwdx + dvdz * dwdz + dwdy * dwdz);
+  const C s33d = (C(2) * dwdz * dwdz - dudx * dudx - dvdy * dvdy + dudz * dwdx + dvdz * dwdy - C(2) * dudy * dvdx) *
+                 C(1.0 / 3.0);
+  const C sd = s11d * s11d + s22d * s22d + s33d * s33d + C(2) * (s12d * s12d + s13d * s13d + s23d * s23d);
+  const C rs = sqrt(sijsq), rd = sqrt(sd);
+  const C den = sijsq * sijsq * rs + sd * sqrt(rd);
+  const C num = C(3) * cw * cw * sd * rd;
+  return den > C(0) ? num / den : C(0);
+}
+
+}  // namespace rrlbm

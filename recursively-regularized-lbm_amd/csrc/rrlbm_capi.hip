@@ -1,0 +1,657 @@
+// C ABI of the gfx950 RR-LBM library (declarations and reference citations in
+// include/rrlbm.h).  A context owns its device buffers and a HIP stream; every
+// entry point validates its arguments, enqueues work on that stream and
+// returns a status code with a per-context message -- the library never exits
+// the process (the reference aborts through CHECK_CUDA_ERROR, LBMLES.h:89-97).
+#include <errno.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <string>
+#include <utility>
+
+#include "kernels.hpp"
+#include "kernels_ref.hpp"
+#include "rrlbm.h"
+
+using namespace rrlbm;
+
+struct rrlbm_ctx {
+  rrlbm_params p;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t n = 0;
+  int q = 27;
+  Grid grid{};
+  RefFields ref{};
+  double* f_temp = nullptr;
+  void* pop_a = nullptr;  // post-collision populations read by the next step
+  void* pop_b = nullptr;
+  size_t pop_bytes = 0;
+  void* u_a = nullptr;  // LES velocity buffers (compute type)
+  void* u_b = nullptr;
+  double* partials = nullptr;
+  double* diag_out = nullptr;
+  double* profiles = nullptr;
+  uint32_t nblocks = 0;
+  bool packed = false;
+  size_t bytes = 0;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(rrlbm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define CK(ctx, call)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess) return fail((ctx), RRLBM_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define NEED(ctx)                                          \
+  do {                                                     \
+    if (!(ctx)) return fail(nullptr, RRLBM_E_INVALID, "null context"); \
+  } while (0)
+
+size_t storage_size(int s) { return s == RRLBM_STORE_F64 ? 8 : (s == RRLBM_STORE_F32 ? 4 : 2); }
+size_t compute_size(int s) { return s == RRLBM_STORE_F64 ? 8 : 4; }
+
+int dalloc(rrlbm_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  c->bytes += bytes;
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return RRLBM_OK;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+double* field_ptr(rrlbm_ctx* c, rrlbm_field f, size_t* count) {
+  const size_t n = c->n, qn = (size_t)c->q * c->n;
+  switch (f) {
+    case RRLBM_FIELD_F: *count = qn; return c->ref.f;
+    case RRLBM_FIELD_F_TEMP: *count = qn; return c->f_temp;
+    case RRLBM_FIELD_RHO: *count = n; return c->ref.rho;
+    case RRLBM_FIELD_UX: *count = n; return c->ref.ux;
+    case RRLBM_FIELD_UY: *count = n; return c->ref.uy;
+    case RRLBM_FIELD_UZ: *count = n; return c->ref.uz;
+    case RRLBM_FIELD_FORCEX: *count = n; return c->ref.fx;
+    case RRLBM_FIELD_FORCEY: *count = n; return c->ref.fy;
+    case RRLBM_FIELD_FORCEZ: *count = n; return c->ref.fz;
+    case RRLBM_FIELD_TAUT: *count = n; return c->ref.taut;
+  }
+  *count = 0;
+  return nullptr;
+}
+
+bool has_walls(const rrlbm_params& p) { return p.wall_x || p.wall_y || p.wall_z; }
+
+FusedArgs fused_args(rrlbm_ctx* c) {
+  FusedArgs a{};
+  a.src = c->pop_a;
+  a.dst = c->pop_b;
+  a.lx = c->grid.lx; a.ly = c->grid.ly; a.lz = c->grid.lz; a.n = c->n;
+  a.div_lx = c->grid.div_lx; a.div_lxly = c->grid.div_lxly;
+  a.walls = c->grid.walls;
+  a.lid_face = c->p.lid_face;
+  a.wall_rr = c->p.wall_rr;
+  a.fx = c->p.force[0]; a.fy = c->p.force[1]; a.fz = c->p.force[2];
+  a.tau = c->p.tau; a.cw = c->p.cw;
+  a.lid_x = c->p.lid_u[0]; a.lid_y = c->p.lid_u[1]; a.lid_z = c->p.lid_u[2];
+  a.f16_scale = c->p.f16_scale;
+  a.nblocks = c->nblocks;
+  a.xcd_remap = c->p.xcd_remap ? 1u : 0u;
+  a.blocks_per_xcd = (c->nblocks + 7) / 8;
+  a.ref_f = c->ref.f; a.ref_rho = c->ref.rho; a.ref_ux = c->ref.ux; a.ref_uy = c->ref.uy; a.ref_uz = c->ref.uz;
+  a.partials = c->partials;
+  return a;
+}
+
+uint32_t fused_grid(const rrlbm_ctx* c) { return c->p.xcd_remap ? 8u * ((c->nblocks + 7) / 8) : c->nblocks; }
+
+int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
+  const bool les = c->p.use_les && mode == kStep;
+  hipError_t e = launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c), c->stream);
+  if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
+  return RRLBM_OK;
+}
+
+int fused_steps(rrlbm_ctx* c, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    FusedArgs a = fused_args(c);
+    if (c->p.use_les) {
+      if (c->p.les_lagged) {
+        a.u_in = c->u_a;
+        a.u_out = c->u_b;
+        a.write_u = 1;
+      } else {
+        FusedArgs m = a;
+        m.u_out = c->u_a;
+        int rc = launch(c, kMacro, m);
+        if (rc) return rc;
+        a.u_in = c->u_a;
+        a.write_u = 0;
+      }
+    }
+    int rc = launch(c, kStep, a);
+    if (rc) return rc;
+    std::swap(c->pop_a, c->pop_b);
+    if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
+  }
+  return RRLBM_OK;
+}
+
+int finalize_diag(rrlbm_ctx* c, uint32_t nparts, rrlbm_diag* out) {
+  CK(c, launch_diag_finalize(c->partials, nparts, c->diag_out, c->stream));
+  double h[kDiagWidth];
+  CK(c, hipMemcpyAsync(h, c->diag_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  out->mass_excess = h[0];
+  out->u_sum[0] = h[1]; out->u_sum[1] = h[2]; out->u_sum[2] = h[3];
+  out->momentum[0] = h[4]; out->momentum[1] = h[5]; out->momentum[2] = h[6];
+  out->kinetic_energy = h[7];
+  out->nonfinite_cells = h[8];
+  out->ncells = (double)c->n;
+  if (h[8] != 0.0) return fail(c, RRLBM_E_NUMERIC, "%.0f cells hold non-finite values", h[8]);
+  return RRLBM_OK;
+}
+
+int make_dir(rrlbm_ctx* c, const char* dir) {
+  struct stat st;
+  if (stat(dir, &st) == 0) return RRLBM_OK;
+  if (mkdir(dir, 0755) != 0 && errno != EEXIST) return fail(c, RRLBM_E_IO, "cannot create directory %s: %s", dir, strerror(errno));
+  return RRLBM_OK;
+}
+
+int write_fields(rrlbm_ctx* c, const char* path, double* const* dev, int nf, size_t count) {
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(c, RRLBM_E_IO, "cannot open %s: %s", path, strerror(errno));
+  std::string buf(count * sizeof(double), '\0');
+  for (int k = 0; k < nf; ++k) {
+    hipError_t e = hipMemcpyAsync(&buf[0], dev[k], count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      fclose(fp);
+      return fail(c, RRLBM_E_HIP, "device to host copy failed: %s", hipGetErrorString(e));
+    }
+    if (fwrite(buf.data(), sizeof(double), count, fp) != count) {
+      fclose(fp);
+      return fail(c, RRLBM_E_IO, "short write to %s", path);
+    }
+  }
+  if (fclose(fp) != 0) return fail(c, RRLBM_E_IO, "cannot close %s: %s", path, strerror(errno));
+  return RRLBM_OK;
+}
+
+int read_fields(rrlbm_ctx* c, const char* path, double* const* dev, int nf, size_t count) {
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(c, RRLBM_E_IO, "cannot open %s: %s", path, strerror(errno));
+  std::string buf(count * sizeof(double), '\0');
+  for (int k = 0; k < nf; ++k) {
+    size_t got = fread(&buf[0], sizeof(double), count, fp);
+    if (got != count) {
+      fclose(fp);
+      return fail(c, RRLBM_E_IO, "short read from %s: %zu of %zu values", path, got, count);
+    }
+    hipError_t e = hipMemcpyAsync(dev[k], buf.data(), count * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      fclose(fp);
+      return fail(c, RRLBM_E_HIP, "host to device copy failed: %s", hipGetErrorString(e));
+    }
+  }
+  fclose(fp);
+  return RRLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rrlbm_version(void) { return "rrlbm-gfx950 1.0 (abi 1)"; }
+
+int rrlbm_params_default(rrlbm_params* p) {
+  if (!p) return fail(nullptr, RRLBM_E_INVALID, "null params");
+  memset(p, 0, sizeof *p);
+  p->lx = p->ly = p->lz = 64;  // parameters.h:9-11
+  p->npop = 27;                // parameters.h:13
+  p->use_les = 0;              // parameters.h:6
+  p->les_lagged = 1;
+  p->storage = RRLBM_STORE_F32;
+  p->wall_y = 1;
+  p->wall_rr = 1;
+  p->lid_face = -1;
+  const double Re = 180.0, visc = 0.000622;  // globals.cu:57-58
+  const double u0 = 2.0 * Re * visc / p->ly;   // globals.cu:59
+  p->tau = 3.0 * visc + 0.5;                   // globals.cu:60
+  p->force[0] = 2.0 * u0 * u0 / p->ly;         // globals.cu:63
+  p->cw = 0.5;                                 // LES.cu:12
+  p->f16_scale = 4096.0;
+  p->xcd_remap = 1;
+  return RRLBM_OK;
+}
+
+int rrlbm_lattice(int npop, int32_t* c, double* w, int32_t* opp) {
+  if (npop != 27 && npop != 19) return fail(nullptr, RRLBM_E_INVALID, "npop must be 27 or 19");
+  auto fill = [&](auto lat) {
+    using L = decltype(lat);
+    for (int i = 0; i < L::Q; ++i) {
+      if (c) { c[3 * i] = L::cx(i); c[3 * i + 1] = L::cy(i); c[3 * i + 2] = L::cz(i); }
+      if (w) w[i] = L::w(i);
+      if (opp) opp[i] = L::opp(i);
+    }
+  };
+  if (npop == 27) fill(D3Q27{});
+  else fill(D3Q19{});
+  return RRLBM_OK;
+}
+
+int rrlbm_device_count(int* n) {
+  if (!n) return fail(nullptr, RRLBM_E_INVALID, "null output");
+  hipError_t e = hipGetDeviceCount(n);
+  if (e != hipSuccess) {
+    *n = 0;
+    return fail(nullptr, RRLBM_E_HIP, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
+  }
+  return RRLBM_OK;
+}
+
+int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
+  if (!pin || !out) return fail(nullptr, RRLBM_E_INVALID, "null argument");
+  *out = nullptr;
+  rrlbm_params p = *pin;
+  if (p.lx < 1 || p.ly < 1 || p.lz < 1) return fail(nullptr, RRLBM_E_INVALID, "grid dimensions must be >= 1");
+  const uint64_t n64 = (uint64_t)p.lx * p.ly * p.lz;
+  if (n64 >= (1ull << 31)) return fail(nullptr, RRLBM_E_INVALID, "grid of %llu cells exceeds 2^31", (unsigned long long)n64);
+  if (p.npop != 27 && p.npop != 19) return fail(nullptr, RRLBM_E_INVALID, "npop must be 27 or 19, got %d", p.npop);
+  if (p.use_les && p.npop != 27) return fail(nullptr, RRLBM_E_UNSUPPORTED, "WALE LES is provided for D3Q27 only");
+  if (p.storage < 0 || p.storage > 2) return fail(nullptr, RRLBM_E_INVALID, "storage must be 0 (F64), 1 (F32) or 2 (F16)");
+  if (!(p.tau > 0.5)) return fail(nullptr, RRLBM_E_INVALID, "tau must exceed 0.5, got %g", p.tau);
+  if ((p.wall_x && p.lx < 2) || (p.wall_y && p.ly < 2) || (p.wall_z && p.lz < 2))
+    return fail(nullptr, RRLBM_E_INVALID, "a walled axis needs at least 2 cells");
+  if (p.lid_face < -1 || p.lid_face > 5) return fail(nullptr, RRLBM_E_INVALID, "lid_face must be -1 or 0..5");
+  if (p.lid_face >= 0) {
+    const int ax = p.lid_face / 2;
+    const int walled = ax == 0 ? p.wall_x : (ax == 1 ? p.wall_y : p.wall_z);
+    if (!walled) return fail(nullptr, RRLBM_E_INVALID, "lid_face %d lies on a periodic axis", p.lid_face);
+  }
+  if (p.f16_scale == 0.0) p.f16_scale = 4096.0;
+  if (!(p.f16_scale > 0.0)) return fail(nullptr, RRLBM_E_INVALID, "f16_scale must be positive");
+  const uint64_t per_pop = n64 * storage_size(p.storage);
+  if (per_pop >= (1ull << 32)) return fail(nullptr, RRLBM_E_INVALID, "one population array exceeds 4 GiB");
+
+  rrlbm_ctx* c = new rrlbm_ctx();
+  c->p = p;
+  c->n = (uint32_t)n64;
+  c->q = p.npop;
+  c->grid.lx = p.lx; c->grid.ly = p.ly; c->grid.lz = p.lz; c->grid.n = c->n;
+  c->grid.div_lx = make_fastdiv(p.lx);
+  c->grid.div_lxly = make_fastdiv((uint32_t)p.lx * p.ly);
+  c->grid.walls = (p.wall_x ? 1u : 0u) | (p.wall_y ? 2u : 0u) | (p.wall_z ? 4u : 0u);
+  c->grid.lid_face = p.lid_face;
+  c->grid.lid_x = p.lid_u[0]; c->grid.lid_y = p.lid_u[1]; c->grid.lid_z = p.lid_u[2];
+  c->nblocks = (c->n + kBlock - 1) / kBlock;
+
+  auto bail = [&](int rc) {
+    g_create_error = c->err;
+    rrlbm_destroy(c);
+    return rc;
+  };
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) return bail(fail(c, RRLBM_E_HIP, "HIP setup failed: %s", hipGetErrorString(e)));
+
+  const size_t n = c->n, qn = (size_t)c->q * n;
+  int rc = RRLBM_OK;
+  void** slots[] = {(void**)&c->ref.f, (void**)&c->f_temp};
+  for (void** s : slots)
+    if ((rc = dalloc(c, s, qn * sizeof(double)))) return bail(rc);
+  void** fields[] = {(void**)&c->ref.rho, (void**)&c->ref.ux, (void**)&c->ref.uy, (void**)&c->ref.uz,
+                     (void**)&c->ref.fx,  (void**)&c->ref.fy, (void**)&c->ref.fz, (void**)&c->ref.taut};
+  for (void** s : fields)
+    if ((rc = dalloc(c, s, n * sizeof(double)))) return bail(rc);
+  c->pop_bytes = qn * storage_size(p.storage);
+  if ((rc = dalloc(c, &c->pop_a, c->pop_bytes))) return bail(rc);
+  if ((rc = dalloc(c, &c->pop_b, c->pop_bytes))) return bail(rc);
+  if (p.use_les) {
+    if ((rc = dalloc(c, &c->u_a, 3 * n * compute_size(p.storage)))) return bail(rc);
+    if ((rc = dalloc(c, &c->u_b, 3 * n * compute_size(p.storage)))) return bail(rc);
+  }
+  if ((rc = dalloc(c, (void**)&c->partials, (size_t)c->nblocks * kDiagWidth * sizeof(double)))) return bail(rc);
+  if ((rc = dalloc(c, (void**)&c->diag_out, kDiagWidth * sizeof(double)))) return bail(rc);
+  if ((rc = dalloc(c, (void**)&c->profiles, (size_t)p.ly * 10 * sizeof(double)))) return bail(rc);
+  // rho = 1 and the uniform force, as initialize() leaves them
+  e = ref_forcing(c->ref, c->grid, p.force[0], p.force[1], p.force[2], c->stream);
+  if (e == hipSuccess) e = ref_init_fields(c->ref, c->grid, InitSpec{kInitRest, 0, 0, 0, 0, 0, 0}, c->stream);
+  if (e == hipSuccess) e = ref_init_equilibrium(c->q, c->ref, c->grid, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return bail(fail(c, RRLBM_E_HIP, "initial fill failed: %s", hipGetErrorString(e)));
+  *out = c;
+  return RRLBM_OK;
+}
+
+int rrlbm_destroy(rrlbm_ctx* c) {
+  if (!c) return RRLBM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->ref.f, c->f_temp, c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz, c->ref.fx, c->ref.fy,
+                  c->ref.fz, c->ref.taut, c->pop_a, c->pop_b, c->u_a, c->u_b, c->partials, c->diag_out,
+                  c->profiles};
+  for (void* p : ptrs) dfree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RRLBM_OK;
+}
+
+const char* rrlbm_last_error(const rrlbm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int rrlbm_get_params(const rrlbm_ctx* c, rrlbm_params* out) {
+  if (!c || !out) return fail(nullptr, RRLBM_E_INVALID, "null argument");
+  *out = c->p;
+  return RRLBM_OK;
+}
+
+int rrlbm_synchronize(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, hipStreamSynchronize(c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_memory_bytes(const rrlbm_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail(nullptr, RRLBM_E_INVALID, "null argument");
+  *bytes = c->bytes;
+  return RRLBM_OK;
+}
+
+int rrlbm_upload(rrlbm_ctx* c, rrlbm_field f, const double* host, size_t count) {
+  NEED(c);
+  size_t want = 0;
+  double* d = field_ptr(c, f, &want);
+  if (!d) return fail(c, RRLBM_E_INVALID, "unknown field %d", (int)f);
+  if (!host || count != want) return fail(c, RRLBM_E_INVALID, "field %d holds %zu values, got %zu", (int)f, want, count);
+  CK(c, hipMemcpyAsync(d, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_download(rrlbm_ctx* c, rrlbm_field f, double* host, size_t count) {
+  NEED(c);
+  size_t want = 0;
+  double* d = field_ptr(c, f, &want);
+  if (!d) return fail(c, RRLBM_E_INVALID, "unknown field %d", (int)f);
+  if (!host || count != want) return fail(c, RRLBM_E_INVALID, "field %d holds %zu values, got %zu", (int)f, want, count);
+  CK(c, hipMemcpyAsync(host, d, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_init_arrays(rrlbm_ctx* c, double ustar, double ystar) {
+  NEED(c);
+  if (!(ystar > 0.0)) return fail(c, RRLBM_E_INVALID, "ystar must be positive");
+  CK(c, ref_init_arrays(c->q, c->ref, c->grid, ustar, ystar, c->p.force[0], c->stream));
+  c->packed = false;
+  return RRLBM_OK;
+}
+
+int rrlbm_forcing(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_forcing_gp(rrlbm_ctx* c, int istep, double rand_phase) {
+  NEED(c);
+  CK(c, ref_forcing_gp(c->ref, c->grid, c->p.force[0], istep, rand_phase, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_les_wale(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_collision_rr(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_collision(c->q, c->ref, c->grid, c->p.tau, c->p.use_les, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_streaming(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_streaming(c->q, c->ref.f, c->f_temp, c->grid, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_bb(rrlbm_ctx* c) {
+  NEED(c);
+  if (c->grid.walls) CK(c, ref_bb(c->q, c->ref.f, c->f_temp, c->grid, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_swap_f(rrlbm_ctx* c) {
+  NEED(c);
+  std::swap(c->ref.f, c->f_temp);
+  return RRLBM_OK;
+}
+
+int rrlbm_macrovar(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_macrovar(c->q, c->ref, c->grid, 0, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_rrbc(rrlbm_ctx* c) {
+  NEED(c);
+  if (c->grid.walls) CK(c, ref_rrbc(c->q, c->ref, c->f_temp, c->grid, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_macrovar_bn(rrlbm_ctx* c) {
+  NEED(c);
+  if (c->grid.walls) CK(c, ref_macrovar(c->q, c->ref, c->grid, 1, c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_step_reference(rrlbm_ctx* c, int nsteps) {
+  NEED(c);
+  if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc;
+    if (c->p.use_les && (rc = rrlbm_les_wale(c))) return rc;
+    if ((rc = rrlbm_collision_rr(c))) return rc;
+    if ((rc = rrlbm_streaming(c))) return rc;
+    if ((rc = rrlbm_bb(c))) return rc;
+    if ((rc = rrlbm_swap_f(c))) return rc;
+    if ((rc = rrlbm_macrovar(c))) return rc;
+    if (c->grid.walls && c->p.wall_rr) {
+      if ((rc = rrlbm_rrbc(c))) return rc;
+      if ((rc = rrlbm_macrovar_bn(c))) return rc;
+    }
+  }
+  c->packed = false;
+  return RRLBM_OK;
+}
+
+static int init_common(rrlbm_ctx* c, const InitSpec& s) {
+  CK(c, ref_init_fields(c->ref, c->grid, s, c->stream));
+  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream));
+  CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  c->packed = false;
+  return RRLBM_OK;
+}
+
+int rrlbm_init_rest(rrlbm_ctx* c) {
+  NEED(c);
+  return init_common(c, InitSpec{kInitRest, 0, 0, 0, 0, 0, 0});
+}
+
+int rrlbm_init_taylor_green_2d(rrlbm_ctx* c, double u0) {
+  NEED(c);
+  return init_common(c, InitSpec{kInitTaylorGreen2D, u0, 0, 0, 0, 0, 0});
+}
+
+int rrlbm_init_taylor_green_3d(rrlbm_ctx* c, double u0) {
+  NEED(c);
+  return init_common(c, InitSpec{kInitTaylorGreen3D, u0, 0, 0, 0, 0, 0});
+}
+
+int rrlbm_init_channel(rrlbm_ctx* c, double u_tau, double visc, uint64_t seed, double amp_streak, double amp_noise) {
+  NEED(c);
+  if (!(u_tau > 0.0) || !(visc > 0.0)) return fail(c, RRLBM_E_INVALID, "u_tau and visc must be positive");
+  // bulk velocity of the mean profile (host, FP64)
+  const double ystar = visc / u_tau;
+  double ub = 0.0;
+  const int ly = c->p.ly;
+  for (int iy = 0; iy < ly; ++iy) {
+    const double d = iy < ly / 2 ? iy + 0.5 : ly - iy - 0.5;
+    const double yp = d / ystar;
+    ub += u_tau * (yp <= 10.8 ? yp : log(yp) / 0.41 + 5.0);
+  }
+  ub /= ly;
+  InitSpec s{kInitChannel, u_tau, visc, amp_streak, amp_noise, ub, seed};
+  return init_common(c, s);
+}
+
+int rrlbm_init_equilibrium(rrlbm_ctx* c) {
+  NEED(c);
+  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream));
+  c->packed = false;
+  return RRLBM_OK;
+}
+
+int rrlbm_fused_pack(rrlbm_ctx* c) {
+  NEED(c);
+  if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
+  CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->p.tau, c->p.use_les, c->p.force[0],
+                 c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream));
+  if (c->p.use_les) CK(c, ref_copy_u(c->p.storage, c->ref, c->grid, c->u_a, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  c->packed = true;
+  return RRLBM_OK;
+}
+
+int rrlbm_fused_step(rrlbm_ctx* c, int nsteps) {
+  NEED(c);
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before fused steps");
+  if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
+  return fused_steps(c, nsteps);
+}
+
+int rrlbm_fused_step_timed(rrlbm_ctx* c, int nsteps, double* ms) {
+  NEED(c);
+  if (!ms) return fail(c, RRLBM_E_INVALID, "null output");
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before fused steps");
+  if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
+  CK(c, hipEventRecord(c->ev0, c->stream));
+  int rc = fused_steps(c, nsteps);
+  if (rc) return rc;
+  CK(c, hipEventRecord(c->ev1, c->stream));
+  CK(c, hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  CK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = (double)t;
+  return RRLBM_OK;
+}
+
+int rrlbm_fused_unpack(rrlbm_ctx* c) {
+  NEED(c);
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "no fused state to unpack");
+  int rc = launch(c, kFinish, fused_args(c));
+  if (rc) return rc;
+  CK(c, hipStreamSynchronize(c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_fused_diag(rrlbm_ctx* c, rrlbm_diag* out) {
+  NEED(c);
+  if (!out) return fail(c, RRLBM_E_INVALID, "null output");
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "no fused state");
+  int rc = launch(c, kDiag, fused_args(c));
+  if (rc) return rc;
+  return finalize_diag(c, c->nblocks, out);
+}
+
+int rrlbm_diag_flow(rrlbm_ctx* c, rrlbm_diag* out) {
+  NEED(c);
+  if (!out) return fail(c, RRLBM_E_INVALID, "null output");
+  CK(c, ref_diag(c->ref, c->grid, c->partials, c->stream));
+  return finalize_diag(c, c->nblocks, out);
+}
+
+int rrlbm_plane_profiles(rrlbm_ctx* c, double* out, size_t count) {
+  NEED(c);
+  if (!out || count != (size_t)c->p.ly * 10) return fail(c, RRLBM_E_INVALID, "profiles need 10*LY = %d values", c->p.ly * 10);
+  CK(c, ref_plane_profiles(c->ref, c->grid, c->p.use_les, c->profiles, c->stream));
+  CK(c, hipMemcpyAsync(out, c->profiles, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  return RRLBM_OK;
+}
+
+int rrlbm_write_flow(rrlbm_ctx* c, const char* dir, int istep) {
+  NEED(c);
+  if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
+  int rc = make_dir(c, dir);
+  if (rc) return rc;
+  char path[1024];
+  snprintf(path, sizeof path, "%s/%09d.dat", dir, istep);
+  double* f[4] = {c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz};
+  return write_fields(c, path, f, 4, c->n);
+}
+
+int rrlbm_write_f(rrlbm_ctx* c, const char* dir, int istep) {
+  NEED(c);
+  if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
+  int rc = make_dir(c, dir);
+  if (rc) return rc;
+  char path[1024];
+  snprintf(path, sizeof path, "%s/f%09d.dat", dir, istep);
+  double* f[1] = {c->ref.f};
+  return write_fields(c, path, f, 1, (size_t)c->q * c->n);
+}
+
+int rrlbm_read_restart(rrlbm_ctx* c, const char* dir, int istep) {
+  NEED(c);
+  if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
+  char path[1024];
+  snprintf(path, sizeof path, "%s/%09d.dat", dir, istep);
+  double* m[4] = {c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz};
+  int rc = read_fields(c, path, m, 4, c->n);
+  if (rc) return rc;
+  snprintf(path, sizeof path, "%s/f%09d.dat", dir, istep);
+  double* f[1] = {c->ref.f};
+  rc = read_fields(c, path, f, 1, (size_t)c->q * c->n);
+  if (rc) return rc;
+  // initial.cu:262: the force fields are re-imposed after a restart
+  CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  c->packed = false;
+  return RRLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+"""Regenerate the golden fixtures in tests/golden/ (run in the build container).
+
+Fixtures (plain text, data only):
+  tg_d3q27_L100_it20000_fortran_midline.txt
+      columns i, v, p, dvdx, dvdy of the reference's committed Taylor-Green
+      output Taylor_Green/RRBMa0.01L100.txt (D3Q27RR.f90, 100x100x2, 20 000
+      steps; midline = mean of rows j = L/2, L/2+1).
+  tg_d3q19_L50_it10000_fortran_midline.txt
+      the same columns from the reference's Taylor_Green/D3Q19RR.f90 compiled
+      from source and run by oracle/build_oracle.py (oracle/_ref/, first case
+      L = 50, 10 000 steps).
+  tg_d3q27_L100_it20000_oracle_fp64_midline.txt
+      i, v, p of the oracle's pure-FP64 run of the same Taylor-Green case
+      (for the FP64 GPU comparison; the Fortran output carries REAL*4
+      intermediates).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+REF_TG27 = "/root/reference/Taylor_Green/RRBMa0.01L100.txt"
+COLS = "i v p dvdx dvdy"
+
+
+def fortran_midline(src, dst, note):
+    a = np.loadtxt(src, skiprows=2)
+    out = np.column_stack([a[:, 0], a[:, 1], a[:, 3], a[:, 8], a[:, 9]])
+    np.savetxt(dst, out, fmt=["%d", "%.10f", "%.10f", "%.10f", "%.10f"], header=note + "\ncolumns: " + COLS)
+
+
+def oracle_fp64_midline(dst):
+    import oracle
+    L, LZ, it = 100, 2, 20000
+    f, rho, u, v, w, tau = oracle.tg_reference(oracle.D3Q27_FORTRAN, L, LZ, it, real4=False)
+    V = v.reshape(LZ, L, L)
+    R = rho.reshape(LZ, L, L)
+    vm = 0.5 * (V[0, L // 2 - 1, :] + V[0, L // 2, :])
+    pm = 0.5 * ((R[0, L // 2 - 1, :] - 1.0) / 3.0 + (R[0, L // 2, :] - 1.0) / 3.0)
+    np.savetxt(dst, np.column_stack([np.arange(1, L + 1), vm, pm]), fmt=["%d", "%.17e", "%.17e"],
+               header="oracle FP64 Taylor-Green D3Q27 100x100x2, 20000 steps (D3Q27RR.f90 restated)\ncolumns: i v p")
+
+
+def main():
+    fortran_midline(REF_TG27, os.path.join(HERE, "tg_d3q27_L100_it20000_fortran_midline.txt"),
+                    "from Taylor_Green/RRBMa0.01L100.txt (reference output)")
+    ref19 = os.path.join(REPO, "oracle", "_ref", "D3Q19RR33BMa0.01L50.txt")
+    if not os.path.exists(ref19):
+        import build_oracle
+        build_oracle.build_reference_tg19(run=True)
+    fortran_midline(ref19, os.path.join(HERE, "tg_d3q19_L50_it10000_fortran_midline.txt"),
+                    "from Taylor_Green/D3Q19RR.f90 built from source (amdflang -O2, default REAL kind 4)")
+    oracle_fp64_midline(os.path.join(HERE, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+// Host build of the kernels' per-cell arithmetic (csrc/rr_cell.hpp) for CPU
+// tests: the same templates the gfx950 kernels instantiate, compiled with g++,
+// so the moment-space D3Q27 formulation and the per-direction D3Q19 one can be
+// compared with the oracle without a GPU.  Test infrastructure only.
+#include <cmath>
+#include "rr_cell.hpp"
+
+using namespace rrlbm;
+
+template <class L, typename C>
+static void cell(const double* f, double rho, double ux, double uy, double uz, const double* F, double tau_eff,
+                 int mode, int fast, double* out) {
+  C h[L::Q];
+  for (int i = 0; i < L::Q; ++i) h[i] = C(f[i] - L::w(i));
+  Macro<C> q{C(rho), C(rho - 1.0), C(ux), C(uy), C(uz)};
+  const C omega = mode == 0 ? C(1.0 - 1.0 / tau_eff) : C(1);
+  const C gs = mode == 0 ? C(1) : C(-1);
+  C o[L::Q];
+  if constexpr (L::Q == 27) {
+    if (fast) {
+      const Moments<C> m = moments_q27<C>(h);
+      const NeqTensor<C> N = neq_q27<C>(m, q, C(F[0]), C(F[1]), C(F[2]));
+      rr_post_q27<C, true>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(1), o);
+    } else {
+      rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
+    }
+  } else {
+    rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
+  }
+  for (int i = 0; i < L::Q; ++i) out[i] = double(o[i]) + L::w(i);
+}
+
+template <class L>
+static void moments(const double* f, double* m) {
+  double h[L::Q];
+  for (int i = 0; i < L::Q; ++i) h[i] = f[i] - L::w(i);
+  Moments<double> r;
+  if constexpr (L::Q == 27) r = moments_q27<double>(h);
+  else r = moments_generic<L, double>(h);
+  const double v[10] = {r.dr, r.jx, r.jy, r.jz, r.pxx, r.pyy, r.pzz, r.pxy, r.pxz, r.pyz};
+  for (int k = 0; k < 10; ++k) m[k] = v[k];
+}
+
+extern "C" {
+// q: 27 or 19; single: 1 = FP32 arithmetic; fast: 1 = moment-space D3Q27 path
+void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double ux, double uy, double uz,
+                const double* F, double tau_eff, int mode, double* out) {
+  if (q == 27) {
+    if (single) cell<D3Q27, float>(f, rho, ux, uy, uz, F, tau_eff, mode, fast, out);
+    else cell<D3Q27, double>(f, rho, ux, uy, uz, F, tau_eff, mode, fast, out);
+  } else {
+    if (single) cell<D3Q19, float>(f, rho, ux, uy, uz, F, tau_eff, mode, fast, out);
+    else cell<D3Q19, double>(f, rho, ux, uy, uz, F, tau_eff, mode, fast, out);
+  }
+}
+void hc_moments(int q, const double* f, double* m) {
+  if (q == 27) moments<D3Q27>(f, m);
+  else moments<D3Q19>(f, m);
+}
+double hc_wale(const double* g, double cw) {
+  return wale_taut<double>(g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], cw);
+}
+void hc_feq(int q, double rho, double ux, double uy, double uz, double* out) {
+  if (q == 27) {
+    double o[27];
+    feq_shifted<D3Q27, double>(rho, ux, uy, uz, o);
+    for (int i = 0; i < 27; ++i) out[i] = o[i] + D3Q27::w(i);
+  } else {
+    double o[19];
+    feq_shifted<D3Q19, double>(rho, ux, uy, uz, o);
+    for (int i = 0; i < 19; ++i) out[i] = o[i] + D3Q19::w(i);
+  }
+}
+}

@@ -1,0 +1,116 @@
+"""CPU check of the GPU kernels' per-cell arithmetic (csrc/rr_cell.hpp).
+
+tests/hostcheck/rr_cell_host.cpp compiles the same templates the gfx950
+kernels instantiate with g++; these tests compare them with the oracle's
+per-direction restatement of collision.cu / BC.cu.  FP64: 1e-15 absolute on
+populations; FP32 arithmetic: 1e-7 absolute.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import PKG, REPO
+
+HC_SRC = os.path.join(REPO, "tests", "hostcheck", "rr_cell_host.cpp")
+HC_LIB = os.path.join(REPO, "tests", "hostcheck", "librr_cell_host.so")
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(HC_LIB) or os.path.getmtime(HC_LIB) < max(
+            os.path.getmtime(HC_SRC), os.path.getmtime(os.path.join(PKG, "csrc", "rr_cell.hpp"))):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-I", os.path.join(PKG, "csrc"), HC_SRC, "-o", HC_LIB], check=True)
+    lib = ctypes.CDLL(HC_LIB)
+    lib.hc_wale.restype = ctypes.c_double
+    return lib
+
+
+def p(a):
+    return a.ctypes.data_as(_D)
+
+
+def hc_cell(hc, q, single, fast, f, rho, u, F, tau, mode):
+    out = np.zeros(q)
+    F = np.asarray(F, dtype=np.float64)
+    d = ctypes.c_double
+    hc.hc_rr_cell(q, single, fast, p(np.ascontiguousarray(f)), d(rho), d(u[0]), d(u[1]), d(u[2]), p(F), d(tau),
+                  mode, p(out))
+    return out
+
+
+def random_cell(kind, rng, consistent):
+    c, w, _ = oracle.lattice(kind)
+    rho = 1 + 0.03 * rng.standard_normal()
+    u = 0.1 * rng.uniform(-1, 1, 3)
+    f = oracle.feq(kind, rho, *u) + 3e-3 * w * rng.standard_normal(len(w))
+    F = 1e-4 * rng.standard_normal(3)
+    if consistent:
+        rho = f.sum()
+        u = (f @ c + 0.5 * F) / rho
+    return f, rho, u, F
+
+
+@pytest.mark.parametrize("q,kind", [(27, oracle.D3Q27_CUDA), (19, oracle.D3Q19)])
+@pytest.mark.parametrize("fast", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("consistent", [True, False])
+def test_cell_fp64_matches_oracle(hc, q, kind, fast, mode, consistent):
+    if q == 19 and fast:
+        pytest.skip("moment-space path is D3Q27 only")
+    rng = np.random.default_rng(10 + mode)
+    for _ in range(50):
+        f, rho, u, F = random_cell(kind, rng, consistent)
+        tau = rng.uniform(0.5005, 1.5)
+        got = hc_cell(hc, q, 0, fast, f, rho, u, F, tau, mode)
+        ref = oracle.rr_cell(kind, f, rho, *u, F, tau, mode)
+        assert np.abs(got - ref).max() < 1e-15
+
+
+@pytest.mark.parametrize("q,kind", [(27, oracle.D3Q27_CUDA), (19, oracle.D3Q19)])
+def test_cell_fp32_arithmetic(hc, q, kind):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        f, rho, u, F = random_cell(kind, rng, True)
+        got = hc_cell(hc, q, 1, 1, f, rho, u, F, 0.53, 0)
+        ref = oracle.rr_cell(kind, f, rho, *u, F, 0.53, 0)
+        assert np.abs(got - ref).max() < 1e-7
+
+
+@pytest.mark.parametrize("q,kind", [(27, oracle.D3Q27_CUDA), (19, oracle.D3Q19)])
+def test_moments_and_feq(hc, q, kind):
+    rng = np.random.default_rng(4)
+    c, w, _ = oracle.lattice(kind)
+    for _ in range(20):
+        f = w * (1 + 0.1 * rng.standard_normal(q))
+        m = np.zeros(10)
+        hc.hc_moments(q, p(f), p(m))
+        h = f - w
+        assert abs(m[0] - h.sum()) < 1e-16
+        assert np.allclose(m[1:4], c.T @ h, atol=1e-16)
+        assert np.allclose(m[4:7], (c ** 2).T @ h, atol=1e-16)
+        assert np.allclose(m[7:10], [h @ (c[:, 0] * c[:, 1]), h @ (c[:, 0] * c[:, 2]), h @ (c[:, 1] * c[:, 2])],
+                           atol=1e-16)
+        rho, u = 1 + 0.02 * rng.standard_normal(), 0.1 * rng.uniform(-1, 1, 3)
+        out = np.zeros(q)
+        hc.hc_feq(q, ctypes.c_double(rho), ctypes.c_double(u[0]), ctypes.c_double(u[1]), ctypes.c_double(u[2]),
+                  p(out))
+        assert np.abs(out - oracle.feq(kind, rho, *u)).max() < 1e-16
+
+
+def test_wale_kernel_formula(hc):
+    """The kernels' WALE (sqrt forms of the powers, 0/0 -> 0) against the
+    long-double matrix definition used to pin the oracle."""
+    from test_oracle import wale_matrix
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        g = 1e-3 * rng.standard_normal((3, 3))
+        flat = np.array([g[0, 0], g[0, 1], g[0, 2], g[1, 0], g[1, 1], g[1, 2], g[2, 0], g[2, 1], g[2, 2]])
+        got = hc.hc_wale(p(flat), ctypes.c_double(0.5))
+        assert abs(got - wale_matrix(g)) <= 1e-13 * abs(wale_matrix(g)) + 1e-30
+    assert hc.hc_wale(p(np.zeros(9)), ctypes.c_double(0.5)) == 0.0

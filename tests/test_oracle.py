@@ -1,0 +1,342 @@
+"""Pins of the oracle (CPU, no GPU).
+
+Each check is fixed by something other than the oracle's own formula: the
+reference's committed output (Taylor-Green), the reference's Fortran compiled
+from source (D3Q19 Taylor-Green), closed-form lattice moments, conservation
+laws, and an independent matrix formulation of WALE in long double.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+
+KINDS = [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN, oracle.D3Q19]
+CS = 1.0 / np.sqrt(3.0)
+
+
+# ---------------------------------------------------------------- lattice
+@pytest.mark.parametrize("kind", KINDS)
+def test_lattice_moments(kind):
+    c, w, opp = oracle.lattice(kind)
+    q = len(w)
+    assert q == (19 if kind == oracle.D3Q19 else 27)
+    assert abs(w.sum() - 1.0) < 1e-15
+    assert np.allclose(w @ c, 0.0, atol=1e-16)
+    assert np.allclose(np.einsum("i,ia,ib->ab", w, c, c), np.eye(3) / 3.0, atol=1e-16)
+    assert np.allclose(np.einsum("i,ia,ib,ic->abc", w, c, c, c), 0.0, atol=1e-16)
+    assert (c[opp] == -c).all()
+    assert len({tuple(v) for v in c}) == q
+    if q == 27:  # D3Q27 weights factorise over the axes: W(0) = 2/3, W(+-1) = 1/6
+        W = np.where(c == 0, 2.0 / 3.0, 1.0 / 6.0).prod(axis=1)
+        assert np.allclose(w, W, rtol=1e-15)
+
+
+def test_channel_ordering_table():
+    """The CUDA channel order (initial.cu:67-69): faces 1-6, edges 7-18, corners 19-26,
+    with bottom-wall incoming set {2,7,8,12,16,19,20,23,24} (streaming.cu:34-44)."""
+    c, _, opp = oracle.lattice(oracle.D3Q27_CUDA)
+    assert [int(i) for i in np.where(c[:, 1] > 0)[0]] == [2, 7, 8, 12, 16, 19, 20, 23, 24]
+    assert list(opp) == [0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15, 16, 13, 14, 11, 12, 25, 26, 23, 24, 21,
+                         22, 19, 20]
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN])
+def test_hermite_orthogonality_d3q27(kind):
+    """sum_i w_i H_a H_b = delta_ab E[H_a^2] of the Gaussian (1, 1, 2, 2, 1) for the
+    first/off-diagonal/diagonal second/aab third/xyz orders."""
+    _, w, _ = oracle.lattice(kind)
+    H = oracle.hermite(kind)
+    G = np.einsum("i,ia,ib->ab", w, H, H)
+    norms = np.array([1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 1], dtype=float)
+    # diagonal second-order polynomials are mutually orthogonal: E[(x^2-1)(y^2-1)] = 0
+    assert np.allclose(G, np.diag(norms), atol=1e-14)
+
+
+# ---------------------------------------------------------------- equilibrium
+@pytest.mark.parametrize("kind", KINDS)
+def test_feq_moments(kind):
+    rng = np.random.default_rng(0)
+    c, w, _ = oracle.lattice(kind)
+    H = oracle.hermite(kind)
+    for _ in range(20):
+        rho = 1.0 + 0.05 * rng.standard_normal()
+        u = 0.1 * rng.uniform(-1, 1, 3)
+        f = oracle.feq(kind, rho, *u)
+        assert abs(f.sum() - rho) < 1e-14
+        assert np.allclose(f @ c, rho * u, atol=1e-15)
+        assert np.allclose(np.einsum("i,ia,ib->ab", f, c, c), rho * (np.eye(3) / 3 + np.outer(u, u)), atol=1e-15)
+        ut = u / CS
+        a = {"210": ut[0] ** 2 * ut[1], "201": ut[0] ** 2 * ut[2], "021": ut[1] ** 2 * ut[2],
+             "120": ut[1] ** 2 * ut[0], "102": ut[2] ** 2 * ut[0], "012": ut[2] ** 2 * ut[1]}
+        col = {"210": 9, "201": 10, "021": 11, "120": 12, "102": 13, "012": 14}
+        if len(w) == 27:
+            # the full third-order Hermite coefficients are reproduced
+            for k in a:
+                assert abs(f @ H[:, col[k]] - rho * a[k]) < 1e-15
+            assert abs(f @ H[:, 15] - rho * ut[0] * ut[1] * ut[2]) < 1e-15
+            # hence the raw third moments
+            assert abs(f @ (c[:, 0] * c[:, 1] * c[:, 2]) - rho * u.prod()) < 1e-16
+            assert abs(f @ (c[:, 0] ** 2 * c[:, 1]) - rho * (u[0] ** 2 * u[1] + u[1] / 3)) < 1e-16
+        else:
+            # D3Q19: the Malaspinas pairs H+ = H_xxy + H_yzz, H- = H_xxy - H_yzz are reproduced
+            for p, q in (("210", "012"), ("102", "120"), ("021", "201")):
+                hp, hm = H[:, col[p]] + H[:, col[q]], H[:, col[p]] - H[:, col[q]]
+                assert abs(f @ hp - rho * (a[p] + a[q])) < 1e-15
+                assert abs(f @ hm - rho * (a[p] - a[q])) < 1e-15
+
+
+# ---------------------------------------------------------------- collision
+def consistent_cell(kind, rng, F):
+    """Random populations with (rho, u) consistent with them (macrovar)."""
+    c, w, _ = oracle.lattice(kind)
+    rho0 = 1 + 0.02 * rng.standard_normal()
+    u0 = 0.08 * rng.uniform(-1, 1, 3)
+    f = oracle.feq(kind, rho0, *u0) + 2e-3 * w * rng.standard_normal(len(w))
+    rho = f.sum()
+    u = (f @ c + 0.5 * np.asarray(F)) / rho
+    return f, rho, u, c, w
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_collision_conservation(kind):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        F = 1e-4 * rng.standard_normal(3)
+        f, rho, u, c, w = consistent_cell(kind, rng, F)
+        tau = rng.uniform(0.51, 2.0)
+        out = oracle.rr_cell(kind, f, rho, *u, F, tau, 0)
+        assert abs(out.sum() - f.sum()) < 1e-14                           # mass
+        # momentum: j* = rho u + rho F/2 = j + F/2 + rho F/2 (He-type term Fbar feq)
+        assert np.allclose(out @ c, f @ c + 0.5 * np.asarray(F) * (1 + rho), atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_collision_special_cases(kind):
+    rng = np.random.default_rng(2)
+    c, w, _ = oracle.lattice(kind)
+    for _ in range(10):
+        rho = 1 + 0.02 * rng.standard_normal()
+        u = 0.08 * rng.uniform(-1, 1, 3)
+        feq = oracle.feq(kind, rho, *u)
+        zero = (0.0, 0.0, 0.0)
+        # equilibrium in -> equilibrium out (any tau)
+        assert np.allclose(oracle.rr_cell(kind, feq, rho, *u, zero, 0.7, 0), feq, atol=1e-15)
+        # tau = 1 relaxes any state to feq
+        f = feq + 1e-3 * w * rng.standard_normal(len(w))
+        assert np.allclose(oracle.rr_cell(kind, f, rho, *u, zero, 1.0, 0), feq, atol=1e-15)
+        # a regularised state is a fixed point of the projection (omega' = 1, tau -> inf)
+        freg = oracle.rr_cell(kind, f, rho, *u, zero, 1.0, 1)  # RRBC: feq + fneq(A2 of f)
+        again = oracle.rr_cell(kind, freg, rho, *u, zero, 1e300, 0)
+        assert np.allclose(again, freg, atol=1e-15)
+        # RRBC with equilibrium input and a force returns feq - Fbar feq
+        F = 1e-4 * rng.standard_normal(3)
+        Fbar = 1.5 * (c @ F - u @ F)
+        out = oracle.rr_cell(kind, feq, rho, *u, F, 1.0, 1)
+        # the projection sees f + Fbar feq, so fneq = w H2:(proj of Fbar feq)/2 + ...: compare moments
+        assert abs(out.sum() - rho) < 1e-14
+        assert out.shape == feq.shape
+        assert np.isfinite(Fbar).all()
+
+
+# ---------------------------------------------------------------- streaming, BB, macrovar
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q19])
+def test_streaming_is_a_shift(kind):
+    lx, ly, lz = 5, 4, 3
+    g = oracle.Fields(kind, lx, ly, lz)
+    g.f[:] = np.arange(g.f.size, dtype=float).reshape(g.f.shape)
+    g.streaming()
+    c, _, _ = oracle.lattice(kind)
+    F = g.f.reshape(g.q, lz, ly, lx)
+    T = g.ftmp.reshape(g.q, lz, ly, lx)
+    for i in range(g.q):
+        assert (np.roll(F[i], shift=(c[i, 2], c[i, 1], c[i, 0]), axis=(0, 1, 2)) == T[i]).all()
+
+
+def test_bounce_back_same_node():
+    kind = oracle.D3Q27_CUDA
+    g = oracle.Fields(kind, 4, 5, 3)
+    g.f[:] = np.random.default_rng(3).standard_normal(g.f.shape)
+    g.ftmp[:] = -7.0
+    g.bb(walls=(0, 1, 0))
+    c, _, opp = oracle.lattice(kind)
+    F = g.f.reshape(27, 3, 5, 4)
+    T = g.ftmp.reshape(27, 3, 5, 4)
+    for i in range(27):
+        if c[i, 1] > 0:
+            assert (T[i, :, 0, :] == F[opp[i], :, 0, :]).all()
+        if c[i, 1] < 0:
+            assert (T[i, :, 4, :] == F[opp[i], :, 4, :]).all()
+        if c[i, 1] == 0:
+            assert (T[i] == -7.0).all()
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q19])
+def test_closed_box_conserves_mass(kind):
+    """Bounce-back on all six faces (no lid, no RR regularisation, no force):
+    total mass is conserved step after step."""
+    from helpers import random_state
+    g = random_state(kind, 6, 5, 4, seed=4, force=(0.0, 0.0, 0.0))
+    g.step(1, 0.6, walls=(1, 1, 1), wall_rr=0)   # makes rho, u consistent with f
+    m0 = g.f.sum()
+    g.step(30, 0.6, walls=(1, 1, 1), wall_rr=0)
+    assert abs(g.f.sum() - m0) < 1e-11
+
+
+def test_macrovar_closed_form():
+    from helpers import random_state
+    g = random_state(oracle.D3Q27_CUDA, 4, 3, 2, seed=5)
+    g.macrovar()
+    c, _, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    rho = g.f.sum(axis=0)
+    j = c.T @ g.f
+    assert np.allclose(g.rho, rho, atol=1e-15)
+    assert np.allclose(g.ux, (j[0] + 0.5 * g.fx) / rho, atol=1e-16)
+    assert np.allclose(g.uz, (j[2] + 0.5 * g.fz) / rho, atol=1e-16)
+
+
+def test_rrbc_leaves_outgoing_populations():
+    from helpers import random_state
+    g = random_state(oracle.D3Q27_CUDA, 5, 6, 4, seed=6)
+    g.streaming(); g.bb(); g.swap(); g.macrovar()
+    before = g.f.copy()
+    g.rrbc(walls=(0, 1, 0))
+    c, _, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    B = before.reshape(27, 4, 6, 5)
+    A = g.f.reshape(27, 4, 6, 5)
+    assert (A[:, :, 1:5, :] == B[:, :, 1:5, :]).all()          # interior untouched
+    for i in range(27):
+        if c[i, 1] <= 0:
+            assert (A[i, :, 0, :] == B[i, :, 0, :]).all()      # bottom: only c_y > 0 replaced
+        if c[i, 1] >= 0:
+            assert (A[i, :, 5, :] == B[i, :, 5, :]).all()
+
+
+# ---------------------------------------------------------------- WALE
+def wale_matrix(g, cw=0.5):
+    """tau_t from the textbook definition in long double: S = sym(g),
+    S^d = sym(g.g) - tr(g.g) I/3, tau_t = 3 Cw^2 (S^d:S^d)^1.5 / ((S:S)^2.5 + (S^d:S^d)^1.25)."""
+    g = np.asarray(g, dtype=np.longdouble)
+    S = 0.5 * (g + g.T)
+    g2 = g @ g
+    Sd = 0.5 * (g2 + g2.T) - np.trace(g2) / 3 * np.eye(3, dtype=np.longdouble)
+    ss, sd = (S * S).sum(), (Sd * Sd).sum()
+    den = ss ** np.longdouble(2.5) + sd ** np.longdouble(1.25)
+    return 0.0 if den == 0 else float(3 * np.longdouble(cw) ** 2 * sd ** np.longdouble(1.5) / den)
+
+
+def test_wale_linear_fields_against_matrix_form():
+    rng = np.random.default_rng(7)
+    lx, ly, lz = 7, 7, 7
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    X = np.stack([x.ravel(), y.ravel(), z.ravel()]).astype(float)
+    for _ in range(5):
+        A = 1e-3 * rng.standard_normal((3, 3))
+        g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+        U = A @ X + 1e-2 * rng.standard_normal((3, 1))
+        g.ux[:], g.uy[:], g.uz[:] = U
+        g.les_wale(cw=0.5, guard=1)
+        T = g.taut.reshape(lz, ly, lx)
+        expect = wale_matrix(A)
+        inner = T[1:-1, 1:-1, 1:-1]
+        assert np.allclose(inner, expect, rtol=1e-12, atol=1e-18)
+
+
+def test_wale_pure_shear_and_rest():
+    lx, ly, lz = 6, 6, 6
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    y = np.repeat(np.arange(ly), lx)[None, :].repeat(lz, axis=0).ravel().astype(float)
+    g.ux[:] = 1e-3 * y
+    g.les_wale(guard=1)
+    T = g.taut.reshape(lz, ly, lx)
+    assert np.abs(T[:, 1:-1, :]).max() < 1e-30      # S^d = 0 for simple shear
+    g.ux[:] = 0.0
+    g.les_wale(guard=1)
+    assert (g.taut == 0.0).all()                     # guarded 0/0
+    g.les_wale(guard=0)
+    assert np.isnan(g.taut).all()                    # the reference's 0/0 (LES.cu:97)
+
+
+# ---------------------------------------------------------------- forcing, init
+def test_forcing_gp_properties():
+    lx, ly, lz = 24, 32, 16
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    fin = 1e-6
+    g.forcing_gp(fin, 1000, 0.3)        # sin(2 pi 1000/2000) = 0: no perturbation
+    assert np.allclose(g.fx, fin, atol=1e-21) and np.abs(g.fy).max() < 1e-20
+    g.forcing_gp(fin, 500, 0.3)
+    FY = g.fy.reshape(lz, ly, lx)
+    assert np.abs(FY).max() > 0
+    assert np.allclose(FY.sum(axis=0), 0.0, atol=1e-18)   # cos(2 pi 2 z/LZ) averages out in z
+    FX = g.fx.reshape(lz, ly, lx)
+    assert np.allclose(FX[:, 0, :], fin) and np.allclose(FX[:, ly // 2, :], fin)  # outside both bands
+
+
+def test_init_arrays_properties():
+    lx, ly, lz = 16, 12, 8
+    visc = 0.000622
+    u0 = 2.0 * 180.0 * visc / ly
+    ystar = visc / u0
+    fin = 2.0 * u0 * u0 / ly
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    g.init_arrays(u0, ystar, fin)
+    c, _, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    assert np.allclose(g.f.sum(axis=0), 1.0, atol=1e-15)
+    assert np.allclose(c[:, 0] @ g.f, -0.5 * fin, atol=1e-18)       # rest populations shifted by -F/2
+    UX = g.ux.reshape(lz, ly, lx)
+    iy = np.arange(ly)
+    yplus = np.where(iy <= ly // 2, (iy + 0.5) / ystar, (ly - iy - 0.5) / ystar)
+    mean = np.where(yplus <= 10.8, u0 * yplus, u0 * (np.log10(yplus) / 0.41 + 5.0))
+    # the x-z average removes the sin(x + z) perturbation (initial.cu:300)
+    assert np.allclose(UX.mean(axis=(0, 2)), mean, rtol=1e-12)
+
+
+def splitmix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def test_counter_rng():
+    """Known-answer: splitmix64(0) = 0xE220A8397B1DCDAF (Vigna's reference generator)."""
+    assert splitmix64(0) == 0xE220A8397B1DCDAF
+    for seed, stream, ctr in ((1, 10, 5), (2024, 12, 123456)):
+        h = splitmix64(seed ^ splitmix64((stream * 0xD1B54A32D192ED03 + ctr) & ((1 << 64) - 1)))
+        assert oracle.uniform(seed, stream, ctr) == (h >> 11) / 2.0 ** 53
+
+
+# ---------------------------------------------------------------- Taylor-Green fixtures
+def midline(v, rho, L, LZ):
+    V = v.reshape(LZ, L, L)
+    R = rho.reshape(LZ, L, L)
+    vm = 0.5 * (V[0, L // 2 - 1, :] + V[0, L // 2, :])
+    pm = 0.5 * ((R[0, L // 2 - 1, :] - 1) / 3 + (R[0, L // 2, :] - 1) / 3)
+    return vm, pm
+
+
+def test_tg_d3q19_matches_reference_fortran():
+    """Oracle with REAL*4 intermediates vs the reference D3Q19RR.f90 built from
+    source (L = 50, 10 000 steps); the file prints 10 decimals."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "tg_d3q19_L50_it10000_fortran_midline.txt"))
+    f, rho, u, v, w, tau = oracle.tg_reference(oracle.D3Q19, 50, 2, 10000, real4=True)
+    vm, pm = midline(v, rho, 50, 2)
+    assert np.abs(vm - ref[:, 1]).max() < 1.0e-10
+    assert np.abs(pm - ref[:, 2]).max() < 1.0e-10
+
+
+@pytest.mark.skipif(os.environ.get("RRLBM_FAST_TESTS") == "1", reason="long run skipped on request")
+def test_tg_d3q27_matches_reference_output():
+    """Oracle with REAL*4 intermediates vs the reference's committed
+    Taylor_Green/RRBMa0.01L100.txt (100x100x2, 20 000 steps), to its print
+    resolution; the pure-FP64 oracle differs from it by the REAL*4 rounding."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "tg_d3q27_L100_it20000_fortran_midline.txt"))
+    f, rho, u, v, w, tau = oracle.tg_reference(oracle.D3Q27_FORTRAN, 100, 2, 20000, real4=True)
+    vm, pm = midline(v, rho, 100, 2)
+    assert np.abs(vm - ref[:, 1]).max() < 1.0e-10
+    assert np.abs(pm - ref[:, 2]).max() < 1.0e-10
+    fp64 = np.loadtxt(os.path.join(GOLDEN, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
+    assert np.abs(fp64[:, 1] - ref[:, 1]).max() < 3e-6
