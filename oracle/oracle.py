@@ -130,6 +130,10 @@ class Fields:
         load().orc_init_channel_fields(*self.dims, d(ustar), d(visc), ctypes.c_uint64(seed), d(amp_streak),
                                        d(amp_noise), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz))
 
+    def init_tg_fields(self, u0, three_d=False):
+        load().orc_init_tg_fields(_I(int(three_d)), *self.dims, ctypes.c_double(u0), _p(self.rho), _p(self.ux),
+                                  _p(self.uy), _p(self.uz))
+
     def les_wale(self, cw=0.5, guard=1):
         load().orc_les_wale(*self.dims, _p(self.ux), _p(self.uy), _p(self.uz), _p(self.taut), ctypes.c_double(cw),
                             _I(guard))

@@ -890,3 +890,32 @@ void orc_tg_ref_run(int kind, int L, int LZ, int real4, int nsteps, double tau, 
     }
     free(fpost);
 }
+
+/* Taylor-Green initial fields of the configurations (the library's
+ * rrlbm_init_taylor_green_2d / _3d), cell centres x = ix + 1/2:
+ *   2D: u = u0 sin(kx x) cos(ky y), v = -u0 cos(kx x) sin(ky y), w = 0,
+ *       rho = 1 + u0^2/(4 cs^2) (cos 2kx x + cos 2ky y), k = 2 pi / L;
+ *   3D: on [0, 2 pi)^3, u = u0 sin x cos y cos z, v = -u0 cos x sin y cos z,
+ *       w = 0, rho = 1 + (u0^2/16)(cos 2x + cos 2y)(cos 2z + 2)/cs^2. */
+void orc_init_tg_fields(int three_d, int lx, int ly, int lz, double u0, double* rho, double* ux, double* uy,
+                        double* uz) {
+    const double pi2 = 2.0 * acos(-1.0);
+    for (int iz = 0; iz < lz; ++iz)
+        for (int iy = 0; iy < ly; ++iy)
+            for (int ix = 0; ix < lx; ++ix) {
+                long idx = cidx(ix, iy, iz, lx, ly);
+                if (!three_d) {
+                    double kx = pi2 / lx, ky = pi2 / ly, x = ix + 0.5, y = iy + 0.5;
+                    ux[idx] = u0 * sin(kx * x) * cos(ky * y);
+                    uy[idx] = -u0 * cos(kx * x) * sin(ky * y);
+                    uz[idx] = 0.0;
+                    rho[idx] = 1.0 + u0 * u0 / (4.0 / 3.0) * (cos(2.0 * kx * x) + cos(2.0 * ky * y));
+                } else {
+                    double x = pi2 * (ix + 0.5) / lx, y = pi2 * (iy + 0.5) / ly, z = pi2 * (iz + 0.5) / lz;
+                    ux[idx] = u0 * sin(x) * cos(y) * cos(z);
+                    uy[idx] = -u0 * cos(x) * sin(y) * cos(z);
+                    uz[idx] = 0.0;
+                    rho[idx] = 1.0 + 3.0 * (u0 * u0 / 16.0) * (cos(2.0 * x) + cos(2.0 * y)) * (cos(2.0 * z) + 2.0);
+                }
+            }
+}

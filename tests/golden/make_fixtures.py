@@ -9,6 +9,9 @@ Fixtures (plain text, data only):
       the same columns from the reference's Taylor_Green/D3Q19RR.f90 compiled
       from source and run by oracle/build_oracle.py (oracle/_ref/, first case
       L = 50, 10 000 steps).
+  tg2d_c1_ke_oracle.txt
+      kinetic energy of the 100x100x4 Taylor-Green configuration under the
+      oracle's reference-order step (steps 0, 500, ..., 2000).
   tg_d3q27_L100_it20000_oracle_fp64_midline.txt
       i, v, p of the oracle's pure-FP64 run of the same Taylor-Green case
       (for the FP64 GPU comparison; the Fortran output carries REAL*4
@@ -45,6 +48,25 @@ def oracle_fp64_midline(dst):
                header="oracle FP64 Taylor-Green D3Q27 100x100x2, 20000 steps (D3Q27RR.f90 restated)\ncolumns: i v p")
 
 
+def tg2d_c1_ke(dst):
+    """Kinetic energy of the 100x100x4 Taylor-Green configuration (u0 = 0.01 cs,
+    Re = 100) under the oracle's reference-order step, at steps 0, 500, ..., 2000."""
+    import oracle
+    lx, ly, lz = 100, 100, 4
+    u0 = 0.01 / np.sqrt(3.0)
+    nu = u0 * lx / 100.0
+    tau = 0.5 + 3.0 * nu
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    g.init_tg_fields(u0)
+    g.init_equilibrium()
+    rows = [(0, g.diag()[8])]
+    for k in range(4):
+        g.step(500, tau, walls=(0, 0, 0), wall_rr=0)
+        rows.append((500 * (k + 1), g.diag()[8]))
+    np.savetxt(dst, np.array(rows), fmt=["%d", "%.17e"],
+               header="oracle FP64 kinetic energy, Taylor-Green 100x100x4, u0 = 0.01 cs, Re = 100\ncolumns: step KE")
+
+
 def main():
     fortran_midline(REF_TG27, os.path.join(HERE, "tg_d3q27_L100_it20000_fortran_midline.txt"),
                     "from Taylor_Green/RRBMa0.01L100.txt (reference output)")
@@ -55,6 +77,7 @@ def main():
     fortran_midline(ref19, os.path.join(HERE, "tg_d3q19_L50_it10000_fortran_midline.txt"),
                     "from Taylor_Green/D3Q19RR.f90 built from source (amdflang -O2, default REAL kind 4)")
     oracle_fp64_midline(os.path.join(HERE, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
+    tg2d_c1_ke(os.path.join(HERE, "tg2d_c1_ke_oracle.txt"))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,154 @@
+"""GPU runs of whole configurations against fixtures, the oracle and analytic
+solutions; initialisers, diagnostics and restart files."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from helpers import fortran_to_cuda_order, maxabs, random_state, upload_state
+
+pytestmark = pytest.mark.gpu
+
+CS = 1.0 / np.sqrt(3.0)
+
+
+def test_taylor_green_reference_mode_fp64(rr):
+    """The reference Taylor-Green case (Taylor_Green/D3Q27RR.f90: 100x100x2,
+    feq + fneq(FD gradients) start, 20 000 steps) on the fused FP64 path:
+    midline v and p equal the oracle's FP64 run to 1e-13 and the reference's
+    REAL*4-intermediate output to its REAL*4 rounding (3e-6)."""
+    L, LZ, it = 100, 2, 20000
+    f, rho, u, v, w, tau = oracle.tg_reference(oracle.D3Q27_FORTRAN, L, LZ, 0, real4=False)
+    with rr.Solver(lx=L, ly=L, lz=LZ, npop=27, storage=0, tau=tau, force=(0, 0, 0), wall_y=0, wall_rr=0) as s:
+        s.upload(rr.Field.F, fortran_to_cuda_order(f, L, LZ))
+        for fld, a in ((rr.Field.RHO, rho), (rr.Field.UX, u), (rr.Field.UY, v), (rr.Field.UZ, w)):
+            s.upload(fld, a)
+        s.fused_pack()
+        s.fused_step(it - 1)
+        s.fused_unpack()
+        R, U, V, W = s.macroscopic()
+    Vg, Rg = V.reshape(LZ, L, L), R.reshape(LZ, L, L)
+    vm = 0.5 * (Vg[0, L // 2 - 1] + Vg[0, L // 2])
+    pm = 0.5 * ((Rg[0, L // 2 - 1] - 1) / 3 + (Rg[0, L // 2] - 1) / 3)
+    fp64 = np.loadtxt(os.path.join(GOLDEN, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
+    fort = np.loadtxt(os.path.join(GOLDEN, "tg_d3q27_L100_it20000_fortran_midline.txt"))
+    assert np.abs(vm - fp64[:, 1]).max() < 1e-13
+    assert np.abs(pm - fp64[:, 2]).max() < 1e-13
+    assert np.abs(vm - fort[:, 1]).max() < 3e-6
+    assert np.abs(Vg[0] - Vg[1]).max() == 0.0            # z-invariant
+
+
+def tg2d_run(rr, storage, steps_list, lx=100, ly=100, lz=4):
+    u0 = 0.01 * CS
+    nu = u0 * lx / 100.0
+    tau = 0.5 + 3.0 * nu
+    ke = []
+    with rr.Solver(lx=lx, ly=ly, lz=lz, npop=27, storage=storage, tau=tau, force=(0, 0, 0), wall_y=0,
+                   wall_rr=0) as s:
+        s.init_taylor_green_2d(u0)
+        ke.append(s.diag_flow()["kinetic_energy"])
+        s.fused_pack()
+        done = 1
+        for target in steps_list:
+            s.fused_step(target - done)
+            done = target
+            ke.append(s.fused_diag()["kinetic_energy"])
+    return np.array(ke), nu
+
+
+def test_taylor_green_c1_energy_decay(rr):
+    """100x100x4 quasi-2D Taylor-Green, Ma = 0.01, Re = 100, 2000 steps:
+    FP64 GPU vs the FP64 oracle (fixture), vs the analytic decay
+    exp(-4 k^2 nu t), and FP32 / FP16 storage vs FP64 within 1e-4 relative."""
+    steps = [500, 1000, 1500, 2000]
+    ke64, nu = tg2d_run(rr, 0, steps)
+    ref = np.loadtxt(os.path.join(GOLDEN, "tg2d_c1_ke_oracle.txt"))
+    assert np.allclose(ke64, ref[:, 1], rtol=1e-11, atol=0)
+    k = 2 * np.pi / 100
+    # pack = collision 1; after fused_step(m) the diagnostics pass streams the
+    # state of step 1 + m, so the entries are the states after `steps` steps
+    t = np.array([0] + steps, dtype=float)
+    analytic = np.exp(-4 * k * k * nu * t)
+    assert np.abs(ke64 / ke64[0] - analytic).max() < 2e-3
+    ke32, _ = tg2d_run(rr, 1, steps)
+    assert np.abs(ke32 / ke64 - 1).max() < 1e-4
+    ke16, _ = tg2d_run(rr, 2, steps)
+    assert np.abs(ke16 / ke64 - 1).max() < 1e-2
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_init_taylor_green_matches_oracle(rr, three_d):
+    dims = (16, 12, 8)
+    g = oracle.Fields(oracle.D3Q27_CUDA, *dims)
+    g.init_tg_fields(0.05, three_d=three_d)
+    g.init_equilibrium()
+    with rr.Solver(lx=dims[0], ly=dims[1], lz=dims[2], storage=0) as s:
+        (s.init_taylor_green_3d if three_d else s.init_taylor_green_2d)(0.05)
+        got = s.macroscopic()
+        gf = s.download(rr.Field.F)
+    for a, b in zip(got, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-15      # device libm vs glibc: ulp-level
+    assert maxabs(gf, g.f) < 1e-15
+
+
+def test_init_channel_matches_oracle(rr):
+    dims = (24, 16, 12)
+    g = oracle.Fields(oracle.D3Q27_CUDA, *dims)
+    g.init_channel_fields(0.0045, 0.0016, 2024, 0.1, 0.01)
+    g.init_equilibrium()
+    with rr.Solver(lx=dims[0], ly=dims[1], lz=dims[2], storage=0) as s:
+        s.init_channel(0.0045, 0.0016, 2024, 0.1, 0.01)
+        got = s.macroscopic()
+        gf = s.download(rr.Field.F)
+    for a, b in zip(got, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-15      # device libm vs glibc: ulp-level
+    assert maxabs(gf, g.f) < 1e-15
+
+
+def test_diag_flow_and_profiles(rr):
+    dims = (20, 14, 10)
+    g = random_state(oracle.D3Q27_CUDA, *dims, seed=21)
+    with rr.Solver(lx=dims[0], ly=dims[1], lz=dims[2], storage=0, use_les=1) as s:
+        upload_state(s, g)
+        d = s.diag_flow()
+        prof = s.output_profile()
+    ref = g.diag()
+    assert np.allclose(d["u_sum"], ref[0:3], rtol=1e-12, atol=1e-14)
+    assert abs(d["mass_excess"] - ref[4]) < 1e-12
+    assert np.allclose(d["momentum"], ref[5:8], rtol=1e-12, atol=1e-14)
+    assert abs(d["kinetic_energy"] - ref[8]) < 1e-12 * abs(ref[8])
+    assert np.allclose(prof, g.profiles(with_taut=True), rtol=1e-12, atol=1e-16)
+
+
+def test_nonfinite_detected(rr):
+    dims = (8, 8, 8)
+    g = random_state(oracle.D3Q27_CUDA, *dims, seed=22)
+    g.rho[17] = np.nan
+    with rr.Solver(lx=8, ly=8, lz=8, storage=0) as s:
+        upload_state(s, g)
+        with pytest.raises(rr.RRLBMError) as ei:
+            s.diag_flow()
+        assert ei.value.status == 4
+
+
+def test_restart_files_roundtrip(rr, tmp_path):
+    dims = (12, 10, 6)
+    g = random_state(oracle.D3Q27_CUDA, *dims, seed=23, force=(1e-5, 0, 0))
+    d = str(tmp_path / "data")
+    with rr.Solver(lx=12, ly=10, lz=6, storage=0, force=(1e-5, 0, 0)) as s:
+        upload_state(s, g)
+        s.write_flow(d, 1234)
+        s.write_f(d, 1234)
+    raw = np.fromfile(os.path.join(d, "%09d.dat" % 1234))
+    assert raw.size == 4 * g.n and np.array_equal(raw[g.n:2 * g.n], g.ux)
+    assert np.array_equal(np.fromfile(os.path.join(d, "f%09d.dat" % 1234)), g.f.ravel())
+    with rr.Solver(lx=12, ly=10, lz=6, storage=0, force=(1e-5, 0, 0)) as s2:
+        s2.read_restart(d, 1234)
+        assert np.array_equal(s2.download(rr.Field.F), g.f)
+        assert np.array_equal(s2.download(rr.Field.UY), g.uy)
+        assert np.all(s2.download(rr.Field.FORCEX) == 1e-5)
+        with pytest.raises(rr.RRLBMError) as ei:
+            s2.read_restart(d, 999)
+        assert ei.value.status == 5
