@@ -208,6 +208,48 @@ int rrlbm_write_flow(rrlbm_ctx* ctx, const char* dir, int istep);
 int rrlbm_write_f(rrlbm_ctx* ctx, const char* dir, int istep);
 int rrlbm_read_restart(rrlbm_ctx* ctx, const char* dir, int istep);
 
+/* Frees the reference-layout FP64 fields of a context whose state lives in
+ * the fused buffers (saves (2*NPOP + 8) * 8 bytes per cell); calls that need
+ * those fields then return RRLBM_E_STATE.  No reference counterpart. */
+int rrlbm_release_reference(rrlbm_ctx* ctx);
+
+/* ------------------------------------------------------- batched ensembles */
+/* Many periodic D3Q27 boxes of uneven size, relaxation time and velocity
+ * stepped together (no reference counterpart: the reference runs one domain
+ * per process).  One launch per step covers every instance: a workgroup reads
+ * its instance from a block -> instance table. */
+typedef struct rrlbm_batch rrlbm_batch;
+typedef struct rrlbm_instance {
+    int32_t lx, ly, lz;
+    int32_t reserved;
+    double tau;   /* 3 nu + 1/2 of this instance */
+    double u0;    /* initial Taylor-Green amplitude */
+} rrlbm_instance;
+/* keep_reference = 0 frees each instance's FP64 reference fields after packing. */
+int rrlbm_batch_create(const rrlbm_instance* inst, int count, int storage, int keep_reference, rrlbm_batch** out);
+int rrlbm_batch_destroy(rrlbm_batch* b);
+const char* rrlbm_batch_last_error(const rrlbm_batch* b);
+/* Total cells over all instances. */
+int rrlbm_batch_cells(const rrlbm_batch* b, int64_t* cells);
+/* Every instance: 3D Taylor-Green fields with its u0, RR equilibrium, packed. */
+int rrlbm_batch_init_taylor_green_3d(rrlbm_batch* b);
+/* nsteps, one batched launch per step. */
+int rrlbm_batch_step(rrlbm_batch* b, int nsteps);
+/* nsteps, one launch per instance per step, in order on one stream. */
+int rrlbm_batch_step_sequential(rrlbm_batch* b, int nsteps);
+/* Capture nsteps batched launches into a HIP graph (replaces a previous one). */
+int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps);
+/* Launch the captured graph `times` times (times * nsteps steps). */
+int rrlbm_batch_graph_launch(rrlbm_batch* b, int times);
+/* Time nsteps (mode 0 batched launches, 1 sequential launches) or `nsteps`
+ * graph replays (mode 2) with HIP events on the batch stream. */
+int rrlbm_batch_time(rrlbm_batch* b, int mode, int nsteps, double* ms);
+/* FP64 diagnostics of one instance's current state. */
+int rrlbm_batch_diag(rrlbm_batch* b, int instance, rrlbm_diag* out);
+/* Borrowed context of one instance (valid until rrlbm_batch_destroy), e.g. to
+ * unpack and download its fields. */
+int rrlbm_batch_context(rrlbm_batch* b, int instance, rrlbm_ctx** ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,13 +142,11 @@ RRLBM_INLINE __device__ C ddir(const void* u, uint32_t n, int comp, uint32_t c0,
   return C(0.5) * (ld_u<C>(u, n, comp, cp) - ld_u<C>(u, n, comp, cm));
 }
 
+// One workgroup's cells of one step: workgroup b of the domain described by a.
 template <int S, class L, bool WALLS, bool LES, int MODE>
-__global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
+RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
   constexpr int Q = L::Q;
-  uint32_t b = blockIdx.x;
-  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
-  if (b >= a.nblocks) return;
   const uint32_t raw = b * kBlock + threadIdx.x;
   const bool active = raw < a.n;
   const uint32_t cell = active ? raw : a.n - 1;
@@ -287,6 +285,50 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
       });
     }
   }
+}
+
+template <int S, class L, bool WALLS, bool LES, int MODE>
+__global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
+  uint32_t b = blockIdx.x;
+  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  if (b >= a.nblocks) return;
+  fused_body<S, L, WALLS, LES, MODE>(a, b);
+}
+
+// Batched step over many periodic D3Q27 instances (ensembles): workgroup g of
+// the launch belongs to instance blk2inst[g]; each instance's domain,
+// relaxation time and A/B buffers come from its FusedArgs in `table`
+// (parity 1 swaps src and dst, so one captured graph alternates buffers).
+template <int S>
+__global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
+                                                         const uint32_t* __restrict__ blk2inst,
+                                                         const uint32_t* __restrict__ first_block,
+                                                         uint32_t total_blocks, uint32_t blocks_per_xcd,
+                                                         uint32_t parity) {
+  uint32_t g = blockIdx.x;
+  g = (g & 7u) * blocks_per_xcd + (g >> 3);
+  if (g >= total_blocks) return;
+  const uint32_t inst = blk2inst[g];
+  FusedArgs a = table[inst];
+  if (parity) {
+    const void* t = a.src;
+    a.src = a.dst;
+    a.dst = const_cast<void*>(t);
+  }
+  fused_body<S, D3Q27, false, false, kStep>(a, g - first_block[inst]);
+}
+
+hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
+                          uint32_t total_blocks, uint32_t parity, hipStream_t s) {
+  const uint32_t per = (total_blocks + 7) / 8;
+  const dim3 grid(8 * per), block(kBlock);
+  switch (storage) {
+    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
+    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
+    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // Deterministic final sum of the per-workgroup diagnostics (fixed order).

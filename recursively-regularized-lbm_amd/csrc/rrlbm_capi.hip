@@ -12,6 +12,7 @@
 
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "kernels.hpp"
 #include "kernels_ref.hpp"
@@ -39,6 +40,7 @@ struct rrlbm_ctx {
   double* profiles = nullptr;
   uint32_t nblocks = 0;
   bool packed = false;
+  bool ref_ok = true;  // reference-layout fields allocated
   size_t bytes = 0;
   std::string err;
 };
@@ -69,6 +71,12 @@ int fail(rrlbm_ctx* c, int code, const char* fmt, ...) {
 #define NEED(ctx)                                          \
   do {                                                     \
     if (!(ctx)) return fail(nullptr, RRLBM_E_INVALID, "null context"); \
+  } while (0)
+
+#define NEED_REF(ctx)                                                                          \
+  do {                                                                                         \
+    NEED(ctx);                                                                                 \
+    if (!(ctx)->ref_ok) return fail((ctx), RRLBM_E_STATE, "reference fields were released"); \
   } while (0)
 
 size_t storage_size(int s) { return s == RRLBM_STORE_F64 ? 8 : (s == RRLBM_STORE_F32 ? 4 : 2); }
@@ -387,8 +395,22 @@ int rrlbm_memory_bytes(const rrlbm_ctx* c, size_t* bytes) {
   return RRLBM_OK;
 }
 
-int rrlbm_upload(rrlbm_ctx* c, rrlbm_field f, const double* host, size_t count) {
+int rrlbm_release_reference(rrlbm_ctx* c) {
   NEED(c);
+  if (!c->ref_ok) return RRLBM_OK;
+  CK(c, hipStreamSynchronize(c->stream));
+  void* ptrs[] = {c->ref.f, c->f_temp, c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz,
+                  c->ref.fx, c->ref.fy, c->ref.fz, c->ref.taut};
+  for (void* p : ptrs) dfree(p);
+  c->bytes -= ((size_t)2 * c->q + 8) * c->n * sizeof(double);
+  c->ref = RefFields{};
+  c->f_temp = nullptr;
+  c->ref_ok = false;
+  return RRLBM_OK;
+}
+
+int rrlbm_upload(rrlbm_ctx* c, rrlbm_field f, const double* host, size_t count) {
+  NEED_REF(c);
   size_t want = 0;
   double* d = field_ptr(c, f, &want);
   if (!d) return fail(c, RRLBM_E_INVALID, "unknown field %d", (int)f);
@@ -399,7 +421,7 @@ int rrlbm_upload(rrlbm_ctx* c, rrlbm_field f, const double* host, size_t count) 
 }
 
 int rrlbm_download(rrlbm_ctx* c, rrlbm_field f, double* host, size_t count) {
-  NEED(c);
+  NEED_REF(c);
   size_t want = 0;
   double* d = field_ptr(c, f, &want);
   if (!d) return fail(c, RRLBM_E_INVALID, "unknown field %d", (int)f);
@@ -410,7 +432,7 @@ int rrlbm_download(rrlbm_ctx* c, rrlbm_field f, double* host, size_t count) {
 }
 
 int rrlbm_init_arrays(rrlbm_ctx* c, double ustar, double ystar) {
-  NEED(c);
+  NEED_REF(c);
   if (!(ystar > 0.0)) return fail(c, RRLBM_E_INVALID, "ystar must be positive");
   CK(c, ref_init_arrays(c->q, c->ref, c->grid, ustar, ystar, c->p.force[0], c->stream));
   c->packed = false;
@@ -418,67 +440,67 @@ int rrlbm_init_arrays(rrlbm_ctx* c, double ustar, double ystar) {
 }
 
 int rrlbm_forcing(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_forcing_gp(rrlbm_ctx* c, int istep, double rand_phase) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_forcing_gp(c->ref, c->grid, c->p.force[0], istep, rand_phase, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_les_wale(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_collision_rr(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_collision(c->q, c->ref, c->grid, c->p.tau, c->p.use_les, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_streaming(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_streaming(c->q, c->ref.f, c->f_temp, c->grid, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_bb(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   if (c->grid.walls) CK(c, ref_bb(c->q, c->ref.f, c->f_temp, c->grid, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_swap_f(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   std::swap(c->ref.f, c->f_temp);
   return RRLBM_OK;
 }
 
 int rrlbm_macrovar(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_macrovar(c->q, c->ref, c->grid, 0, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_rrbc(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   if (c->grid.walls) CK(c, ref_rrbc(c->q, c->ref, c->f_temp, c->grid, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_macrovar_bn(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   if (c->grid.walls) CK(c, ref_macrovar(c->q, c->ref, c->grid, 1, c->stream));
   return RRLBM_OK;
 }
 
 int rrlbm_step_reference(rrlbm_ctx* c, int nsteps) {
-  NEED(c);
+  NEED_REF(c);
   if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
   for (int s = 0; s < nsteps; ++s) {
     int rc;
@@ -507,22 +529,22 @@ static int init_common(rrlbm_ctx* c, const InitSpec& s) {
 }
 
 int rrlbm_init_rest(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   return init_common(c, InitSpec{kInitRest, 0, 0, 0, 0, 0, 0});
 }
 
 int rrlbm_init_taylor_green_2d(rrlbm_ctx* c, double u0) {
-  NEED(c);
+  NEED_REF(c);
   return init_common(c, InitSpec{kInitTaylorGreen2D, u0, 0, 0, 0, 0, 0});
 }
 
 int rrlbm_init_taylor_green_3d(rrlbm_ctx* c, double u0) {
-  NEED(c);
+  NEED_REF(c);
   return init_common(c, InitSpec{kInitTaylorGreen3D, u0, 0, 0, 0, 0, 0});
 }
 
 int rrlbm_init_channel(rrlbm_ctx* c, double u_tau, double visc, uint64_t seed, double amp_streak, double amp_noise) {
-  NEED(c);
+  NEED_REF(c);
   if (!(u_tau > 0.0) || !(visc > 0.0)) return fail(c, RRLBM_E_INVALID, "u_tau and visc must be positive");
   // bulk velocity of the mean profile (host, FP64)
   const double ystar = visc / u_tau;
@@ -539,14 +561,14 @@ int rrlbm_init_channel(rrlbm_ctx* c, double u_tau, double visc, uint64_t seed, d
 }
 
 int rrlbm_init_equilibrium(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream));
   c->packed = false;
   return RRLBM_OK;
 }
 
 int rrlbm_fused_pack(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
   CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->p.tau, c->p.use_les, c->p.force[0],
                  c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream));
@@ -580,7 +602,7 @@ int rrlbm_fused_step_timed(rrlbm_ctx* c, int nsteps, double* ms) {
 }
 
 int rrlbm_fused_unpack(rrlbm_ctx* c) {
-  NEED(c);
+  NEED_REF(c);
   if (!c->packed) return fail(c, RRLBM_E_STATE, "no fused state to unpack");
   int rc = launch(c, kFinish, fused_args(c));
   if (rc) return rc;
@@ -598,14 +620,14 @@ int rrlbm_fused_diag(rrlbm_ctx* c, rrlbm_diag* out) {
 }
 
 int rrlbm_diag_flow(rrlbm_ctx* c, rrlbm_diag* out) {
-  NEED(c);
+  NEED_REF(c);
   if (!out) return fail(c, RRLBM_E_INVALID, "null output");
   CK(c, ref_diag(c->ref, c->grid, c->partials, c->stream));
   return finalize_diag(c, c->nblocks, out);
 }
 
 int rrlbm_plane_profiles(rrlbm_ctx* c, double* out, size_t count) {
-  NEED(c);
+  NEED_REF(c);
   if (!out || count != (size_t)c->p.ly * 10) return fail(c, RRLBM_E_INVALID, "profiles need 10*LY = %d values", c->p.ly * 10);
   CK(c, ref_plane_profiles(c->ref, c->grid, c->p.use_les, c->profiles, c->stream));
   CK(c, hipMemcpyAsync(out, c->profiles, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -614,7 +636,7 @@ int rrlbm_plane_profiles(rrlbm_ctx* c, double* out, size_t count) {
 }
 
 int rrlbm_write_flow(rrlbm_ctx* c, const char* dir, int istep) {
-  NEED(c);
+  NEED_REF(c);
   if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
   int rc = make_dir(c, dir);
   if (rc) return rc;
@@ -625,7 +647,7 @@ int rrlbm_write_flow(rrlbm_ctx* c, const char* dir, int istep) {
 }
 
 int rrlbm_write_f(rrlbm_ctx* c, const char* dir, int istep) {
-  NEED(c);
+  NEED_REF(c);
   if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
   int rc = make_dir(c, dir);
   if (rc) return rc;
@@ -636,7 +658,7 @@ int rrlbm_write_f(rrlbm_ctx* c, const char* dir, int istep) {
 }
 
 int rrlbm_read_restart(rrlbm_ctx* c, const char* dir, int istep) {
-  NEED(c);
+  NEED_REF(c);
   if (!dir) return fail(c, RRLBM_E_INVALID, "null directory");
   char path[1024];
   snprintf(path, sizeof path, "%s/%09d.dat", dir, istep);
@@ -651,6 +673,304 @@ int rrlbm_read_restart(rrlbm_ctx* c, const char* dir, int istep) {
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   c->packed = false;
+  return RRLBM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched ensembles
+struct rrlbm_batch {
+  int storage = RRLBM_STORE_F32;
+  bool keep_reference = true;
+  std::vector<rrlbm_ctx*> inst;
+  std::vector<double> u0;  // initial Taylor-Green amplitude per instance
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  FusedArgs* d_table = nullptr;
+  uint32_t* d_blk2inst = nullptr;
+  uint32_t* d_first = nullptr;
+  uint32_t total_blocks = 0;
+  uint32_t parity = 0;  // 1: every instance's current state sits in its table dst buffer
+  bool ready = false;   // instances initialised and packed, table uploaded
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  uint32_t graph_parity = 0;
+  int64_t cells = 0;
+  std::string err;
+};
+
+namespace {
+
+int bfail(rrlbm_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b)
+    b->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define BCK(b, call)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (call);                                                                               \
+    if (e_ != hipSuccess) return bfail((b), RRLBM_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define NEED_BATCH(b)                                                                         \
+  do {                                                                                        \
+    if (!(b)) return bfail(nullptr, RRLBM_E_INVALID, "null batch");                          \
+    if (!(b)->ready) return bfail((b), RRLBM_E_STATE, "initialise the batch before stepping"); \
+  } while (0)
+
+// After a step of every instance each context's current buffer is the other one.
+void batch_swap_all(rrlbm_batch* b) {
+  for (rrlbm_ctx* c : b->inst) std::swap(c->pop_a, c->pop_b);
+  b->parity ^= 1u;
+}
+
+// Device table: instance k's FusedArgs with src = the buffer holding its state
+// when the batch parity is 0.
+int batch_upload_table(rrlbm_batch* b) {
+  std::vector<FusedArgs> t(b->inst.size());
+  for (size_t k = 0; k < b->inst.size(); ++k) {
+    t[k] = fused_args(b->inst[k]);
+    if (b->parity) {
+      const void* cur = t[k].src;
+      t[k].src = t[k].dst;
+      t[k].dst = const_cast<void*>(cur);
+    }
+  }
+  BCK(b, hipMemcpyAsync(b->d_table, t.data(), t.size() * sizeof(FusedArgs), hipMemcpyHostToDevice, b->stream));
+  BCK(b, hipStreamSynchronize(b->stream));
+  return RRLBM_OK;
+}
+
+int batch_launch(rrlbm_batch* b) {
+  hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, b->parity, b->stream);
+  if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "batched launch failed: %s", hipGetErrorString(e));
+  batch_swap_all(b);
+  return RRLBM_OK;
+}
+
+int batch_sequential(rrlbm_batch* b, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    for (rrlbm_ctx* c : b->inst) {
+      hipError_t e = launch_fused(c->p.storage, 27, false, false, kStep, fused_args(c), fused_grid(c), b->stream);
+      if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "instance launch failed: %s", hipGetErrorString(e));
+    }
+    batch_swap_all(b);
+  }
+  return RRLBM_OK;
+}
+
+int batch_graph_launch(rrlbm_batch* b, int times) {
+  if (!b->exec) return bfail(b, RRLBM_E_STATE, "no captured graph");
+  for (int t = 0; t < times; ++t) {
+    if (b->parity != b->graph_parity)
+      return bfail(b, RRLBM_E_STATE, "the graph was captured at the other buffer parity; capture again");
+    BCK(b, hipGraphLaunch(b->exec, b->stream));
+    for (int s = 0; s < b->graph_steps; ++s) batch_swap_all(b);
+  }
+  return RRLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rrlbm_batch_create(const rrlbm_instance* inst, int count, int storage, int keep_reference, rrlbm_batch** out) {
+  if (!inst || !out || count < 1) return bfail(nullptr, RRLBM_E_INVALID, "need at least one instance");
+  *out = nullptr;
+  rrlbm_batch* b = new rrlbm_batch();
+  b->storage = storage;
+  b->keep_reference = keep_reference != 0;
+  auto bail = [&](int rc) {
+    g_create_error = b->err;
+    rrlbm_batch_destroy(b);
+    return rc;
+  };
+  std::vector<uint32_t> blk2inst, first;
+  for (int k = 0; k < count; ++k) {
+    rrlbm_params p;
+    rrlbm_params_default(&p);
+    p.lx = inst[k].lx; p.ly = inst[k].ly; p.lz = inst[k].lz;
+    p.npop = 27; p.use_les = 0; p.storage = storage;
+    p.wall_x = p.wall_y = p.wall_z = 0; p.wall_rr = 0; p.lid_face = -1;
+    p.tau = inst[k].tau;
+    p.force[0] = p.force[1] = p.force[2] = 0.0;
+    rrlbm_ctx* c = nullptr;
+    int rc = rrlbm_create(&p, &c);
+    if (rc) {
+      b->err = std::string("instance ") + std::to_string(k) + ": " + g_create_error;
+      return bail(rc);
+    }
+    b->inst.push_back(c);
+    first.push_back((uint32_t)blk2inst.size());
+    for (uint32_t g = 0; g < c->nblocks; ++g) blk2inst.push_back((uint32_t)k);
+    b->cells += c->n;
+  }
+  for (int k = 0; k < count; ++k) b->u0.push_back(inst[k].u0);
+  b->total_blocks = (uint32_t)blk2inst.size();
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipMalloc(&b->d_table, count * sizeof(FusedArgs));
+  if (e == hipSuccess) e = hipMalloc(&b->d_blk2inst, blk2inst.size() * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->d_first, first.size() * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_blk2inst, blk2inst.data(), blk2inst.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                       b->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return bail(bfail(b, RRLBM_E_HIP, "batch setup failed: %s", hipGetErrorString(e)));
+  *out = b;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_destroy(rrlbm_batch* b) {
+  if (!b) return RRLBM_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (rrlbm_ctx* c : b->inst) rrlbm_destroy(c);
+  dfree(b->d_table);
+  dfree(b->d_blk2inst);
+  dfree(b->d_first);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return RRLBM_OK;
+}
+
+const char* rrlbm_batch_last_error(const rrlbm_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int rrlbm_batch_cells(const rrlbm_batch* b, int64_t* cells) {
+  if (!b || !cells) return bfail(nullptr, RRLBM_E_INVALID, "null argument");
+  *cells = b->cells;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_init_taylor_green_3d(rrlbm_batch* b) {
+  if (!b) return bfail(nullptr, RRLBM_E_INVALID, "null batch");
+  BCK(b, hipStreamSynchronize(b->stream));
+  for (size_t k = 0; k < b->inst.size(); ++k) {
+    rrlbm_ctx* c = b->inst[k];
+    if (!c->ref_ok) return bfail(b, RRLBM_E_STATE, "instance %zu: reference fields were released", k);
+    int rc = rrlbm_init_taylor_green_3d(c, b->u0[k]);
+    if (!rc) rc = rrlbm_fused_pack(c);
+    if (!rc && !b->keep_reference) rc = rrlbm_release_reference(c);
+    if (rc) return bfail(b, rc, "instance %zu: %s", k, c->err.c_str());
+  }
+  b->parity = 0;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  int rc = batch_upload_table(b);
+  if (rc) return rc;
+  b->ready = true;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_step(rrlbm_batch* b, int nsteps) {
+  NEED_BATCH(b);
+  if (nsteps < 0) return bfail(b, RRLBM_E_INVALID, "nsteps must be >= 0");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = batch_launch(b);
+    if (rc) return rc;
+  }
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_step_sequential(rrlbm_batch* b, int nsteps) {
+  NEED_BATCH(b);
+  if (nsteps < 0) return bfail(b, RRLBM_E_INVALID, "nsteps must be >= 0");
+  return batch_sequential(b, nsteps);
+}
+
+int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps) {
+  NEED_BATCH(b);
+  if (nsteps < 1) return bfail(b, RRLBM_E_INVALID, "nsteps must be >= 1");
+  BCK(b, hipStreamSynchronize(b->stream));
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  const uint32_t start = b->parity;
+  BCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  uint32_t par = start;
+  for (int s = 0; s < nsteps; ++s) {
+    hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, par, b->stream);
+    if (e != hipSuccess) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return bfail(b, RRLBM_E_HIP, "launch during capture failed: %s", hipGetErrorString(e));
+    }
+    par ^= 1u;
+  }
+  BCK(b, hipStreamEndCapture(b->stream, &b->graph));
+  BCK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_steps = nsteps;
+  b->graph_parity = start;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_graph_launch(rrlbm_batch* b, int times) {
+  NEED_BATCH(b);
+  if (times < 0) return bfail(b, RRLBM_E_INVALID, "times must be >= 0");
+  return batch_graph_launch(b, times);
+}
+
+int rrlbm_batch_time(rrlbm_batch* b, int mode, int nsteps, double* ms) {
+  NEED_BATCH(b);
+  if (!ms || nsteps < 0) return bfail(b, RRLBM_E_INVALID, "bad argument");
+  BCK(b, hipStreamSynchronize(b->stream));
+  BCK(b, hipEventRecord(b->ev0, b->stream));
+  int rc = RRLBM_OK;
+  if (mode == 0) rc = rrlbm_batch_step(b, nsteps);
+  else if (mode == 1) rc = batch_sequential(b, nsteps);
+  else if (mode == 2) rc = batch_graph_launch(b, nsteps);
+  else return bfail(b, RRLBM_E_INVALID, "mode must be 0, 1 or 2");
+  if (rc) return rc;
+  BCK(b, hipEventRecord(b->ev1, b->stream));
+  BCK(b, hipEventSynchronize(b->ev1));
+  float t = 0.0f;
+  BCK(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+  *ms = (double)t;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_diag(rrlbm_batch* b, int instance, rrlbm_diag* out) {
+  NEED_BATCH(b);
+  if (instance < 0 || instance >= (int)b->inst.size()) return bfail(b, RRLBM_E_INVALID, "no instance %d", instance);
+  BCK(b, hipStreamSynchronize(b->stream));
+  rrlbm_ctx* c = b->inst[instance];
+  int rc = rrlbm_fused_diag(c, out);
+  if (rc) return bfail(b, rc, "instance %d: %s", instance, c->err.c_str());
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_context(rrlbm_batch* b, int instance, rrlbm_ctx** ctx) {
+  if (!b || !ctx) return bfail(nullptr, RRLBM_E_INVALID, "null argument");
+  if (instance < 0 || instance >= (int)b->inst.size()) return bfail(b, RRLBM_E_INVALID, "no instance %d", instance);
+  BCK(b, hipStreamSynchronize(b->stream));
+  *ctx = b->inst[instance];
   return RRLBM_OK;
 }
 

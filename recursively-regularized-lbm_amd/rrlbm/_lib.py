@@ -33,6 +33,13 @@ class Params(ctypes.Structure):
     ]
 
 
+class Instance(ctypes.Structure):
+    """rrlbm_instance: one periodic D3Q27 box of a batched ensemble."""
+
+    _fields_ = [("lx", ctypes.c_int32), ("ly", ctypes.c_int32), ("lz", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("tau", ctypes.c_double), ("u0", ctypes.c_double)]
+
+
 class Diag(ctypes.Structure):
     """rrlbm_diag: FP64 sums of diag_flow (output.cu:24-35) and conservation checks."""
 
@@ -94,6 +101,18 @@ SIGNATURES = {
     "rrlbm_write_flow": [_P, ctypes.c_char_p, _I],
     "rrlbm_write_f": [_P, ctypes.c_char_p, _I],
     "rrlbm_read_restart": [_P, ctypes.c_char_p, _I],
+    "rrlbm_release_reference": [_P],
+    "rrlbm_batch_create": [ctypes.POINTER(Instance), _I, _I, _I, ctypes.POINTER(_P)],
+    "rrlbm_batch_destroy": [_P],
+    "rrlbm_batch_cells": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "rrlbm_batch_init_taylor_green_3d": [_P],
+    "rrlbm_batch_step": [_P, _I],
+    "rrlbm_batch_step_sequential": [_P, _I],
+    "rrlbm_batch_graph_capture": [_P, _I],
+    "rrlbm_batch_graph_launch": [_P, _I],
+    "rrlbm_batch_time": [_P, _I, _I, _D],
+    "rrlbm_batch_diag": [_P, _I, ctypes.POINTER(Diag)],
+    "rrlbm_batch_context": [_P, _I, ctypes.POINTER(_P)],
 }
 
 _lib = None
@@ -116,6 +135,8 @@ def load():
     lib.rrlbm_version.argtypes = []
     lib.rrlbm_last_error.restype = ctypes.c_char_p
     lib.rrlbm_last_error.argtypes = [_P]
+    lib.rrlbm_batch_last_error.restype = ctypes.c_char_p
+    lib.rrlbm_batch_last_error.argtypes = [_P]
     _lib = lib
     return lib
 
@@ -123,4 +144,10 @@ def load():
 def check(status, ctx=None):
     if status != 0:
         msg = load().rrlbm_last_error(ctx)
+        raise RRLBMError(status, msg.decode() if msg else "")
+
+
+def check_batch(status, batch=None):
+    if status != 0:
+        msg = load().rrlbm_batch_last_error(batch)
         raise RRLBMError(status, msg.decode() if msg else "")

@@ -12,7 +12,7 @@ import enum
 
 import numpy as np
 
-from ._lib import Diag, Params, RRLBMError, check, load
+from ._lib import Diag, Instance, Params, RRLBMError, check, check_batch, load
 
 
 class Storage(enum.IntEnum):
@@ -252,3 +252,95 @@ class Solver:
 
     def read_restart(self, directory, istep):
         self._call("rrlbm_read_restart", directory.encode(), int(istep))
+
+    def release_reference(self):
+        self._call("rrlbm_release_reference")
+
+
+class _Borrowed(Solver):
+    """A Solver view of one batch instance (owned by the batch)."""
+
+    def __init__(self, lib, handle):
+        self._lib = lib
+        self._h = handle
+        self.params = Params()
+        self._call("rrlbm_get_params", ctypes.byref(self.params))
+        self.lx, self.ly, self.lz = self.params.lx, self.params.ly, self.params.lz
+        self.npop = self.params.npop
+        self.n = self.lx * self.ly * self.lz
+
+    def close(self):
+        self._h = None
+
+
+class Batch:
+    """Ensemble of periodic D3Q27 boxes stepped by one launch per step."""
+
+    def __init__(self, instances, storage=Storage.F32, keep_reference=True):
+        self._lib = load()
+        arr = (Instance * len(instances))()
+        for k, (lx, ly, lz, tau, u0) in enumerate(instances):
+            arr[k].lx, arr[k].ly, arr[k].lz, arr[k].tau, arr[k].u0 = lx, ly, lz, tau, u0
+        h = ctypes.c_void_p()
+        check_batch(self._lib.rrlbm_batch_create(arr, len(instances), int(storage), int(keep_reference),
+                                                 ctypes.byref(h)))
+        self._h = h
+        self.count = len(instances)
+
+    def _call(self, name, *args):
+        if self._h is None:
+            raise RRLBMError(-1, "batch is closed")
+        check_batch(getattr(self._lib, name)(self._h, *args), self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.rrlbm_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cells(self):
+        n = ctypes.c_int64(0)
+        self._call("rrlbm_batch_cells", ctypes.byref(n))
+        return n.value
+
+    def init_taylor_green_3d(self):
+        self._call("rrlbm_batch_init_taylor_green_3d")
+
+    def step(self, nsteps):
+        self._call("rrlbm_batch_step", int(nsteps))
+
+    def step_sequential(self, nsteps):
+        self._call("rrlbm_batch_step_sequential", int(nsteps))
+
+    def graph_capture(self, nsteps):
+        self._call("rrlbm_batch_graph_capture", int(nsteps))
+
+    def graph_launch(self, times=1):
+        self._call("rrlbm_batch_graph_launch", int(times))
+
+    def time(self, mode, n):
+        """mode 0: n batched steps, 1: n sequential steps, 2: n graph replays; returns ms."""
+        ms = ctypes.c_double(0.0)
+        self._call("rrlbm_batch_time", int(mode), int(n), ctypes.byref(ms))
+        return ms.value
+
+    def diag(self, k):
+        d = Diag()
+        self._call("rrlbm_batch_diag", int(k), ctypes.byref(d))
+        return d.as_dict()
+
+    def instance(self, k):
+        h = ctypes.c_void_p()
+        self._call("rrlbm_batch_context", int(k), ctypes.byref(h))
+        return _Borrowed(self._lib, h)
