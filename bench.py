@@ -159,6 +159,20 @@ def main_ensemble(args):
     print(json.dumps(line))
 
 
+def main_copy():
+    """Calibration workload for rocprofv3's FETCH_SIZE / WRITE_SIZE counters: a
+    plain device-to-device copy of 1 GiB, five times."""
+    import torch
+
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n, "copies": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,13 +180,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble"])
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         return main_ensemble(args)
+    if args.workload == "copy":
+        return main_copy()
     import rrlbm
 
     rrlbm.load()
