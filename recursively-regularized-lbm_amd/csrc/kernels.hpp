@@ -75,6 +75,7 @@ struct FusedArgs {
   int32_t write_u;      // STEP with LES: also write the velocity to u_out
   double fx, fy, fz;    // uniform body force
   double tau, cw;
+  double omega;         // 1 - 1/tau (without LES)
   double lid_x, lid_y, lid_z;
   double f16_scale;     // FP16: stored value = (f_i - w_i) * f16_scale / w_i
   uint32_t nblocks;     // workgroups covering the domain

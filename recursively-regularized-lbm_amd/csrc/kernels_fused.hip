@@ -23,13 +23,14 @@
 
 namespace rrlbm {
 
+// Population i of the cell at byte offset `off` within one population array
+// (64-bit base of array i + 32-bit byte offset: one saddr global load).
 template <int S>
-RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_t n, int i, uint32_t cell,
-                                                       typename StoreT<S>::C unscale) {
+RRLBM_INLINE __device__ typename StoreT<S>::C load_pop_b(const void* base, uint32_t n, int i, uint32_t off,
+                                                         typename StoreT<S>::C unscale) {
   using T = typename StoreT<S>::T;
   using C = typename StoreT<S>::C;
   const char* p = reinterpret_cast<const char*>(base) + (size_t)i * n * sizeof(T);
-  const uint32_t off = cell * (uint32_t)sizeof(T);
   const T v = *reinterpret_cast<const T*>(p + off);
   if constexpr (S == kF16)
     return C(v) * unscale;
@@ -38,11 +39,19 @@ RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_
 }
 
 template <int S>
+RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_t n, int i, uint32_t cell,
+                                                       typename StoreT<S>::C unscale) {
+  return load_pop_b<S>(base, n, i, cell * (uint32_t)sizeof(typename StoreT<S>::T), unscale);
+}
+
+// Post-collision populations are written once and read by the next launch:
+// non-temporal stores keep them from displacing the gather's cache lines.
+template <int S>
 RRLBM_INLINE __device__ void store_pop(void* base, uint32_t n, int i, uint32_t cell, typename StoreT<S>::C v) {
   using T = typename StoreT<S>::T;
   char* p = reinterpret_cast<char*>(base) + (size_t)i * n * sizeof(T);
   const uint32_t off = cell * (uint32_t)sizeof(T);
-  *reinterpret_cast<T*>(p + off) = T(v);
+  __builtin_nontemporal_store(T(v), reinterpret_cast<T*>(p + off));
 }
 
 // Does population i (velocity x,y,z) enter the domain through face f?
@@ -57,16 +66,16 @@ RRLBM_HD constexpr bool enters(int face, int x, int y, int z) {
   }
 }
 
-template <class L, typename C, int face>
+template <class L, typename C, int face, int FM>
 RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
   Moments<C> m;
   if constexpr (L::Q == 27) m = moments_q27<C>(h);
   else m = moments_generic<L, C>(h);
-  const Macro<C> q = macro_of<C>(m, Fx, Fy, Fz);
+  const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
   C out[L::Q];
   if constexpr (L::Q == 27) {
-    const NeqTensor<C> N = neq_q27<C>(m, q, Fx, Fy, Fz);
-    rr_post_q27<C, true>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
+    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
   } else {
     rr_post_generic<L, C>(h, q, Fx, Fy, Fz, C(1), C(-1), out);
   }
@@ -90,7 +99,7 @@ RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, int i
   return v;
 }
 
-template <int S, class L>
+template <int S, class L, int FM>
 RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
                                                          uint32_t x, uint32_t y, uint32_t z, uint32_t cell) {
   using C = typename StoreT<S>::C;
@@ -121,7 +130,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
         if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, i, face, cell, inv_scale);
       });
     }
-    regularise_face<L, C, face>(h, Fx, Fy, Fz);
+    regularise_face<L, C, face, FM>(h, Fx, Fy, Fz);
     done = true;
   });
 }
@@ -143,7 +152,7 @@ RRLBM_INLINE __device__ C ddir(const void* u, uint32_t n, int comp, uint32_t c0,
 }
 
 // One workgroup's cells of one step: workgroup b of the domain described by a.
-template <int S, class L, bool WALLS, bool LES, int MODE>
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM>
 RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
   constexpr int Q = L::Q;
@@ -160,26 +169,32 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
   const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
   const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  // byte offsets of the source rows and columns: each population's source is
+  // one three-operand add
+  constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
+  const uint32_t row_b = a.lx * sz, plane_b = lxly * sz;
+  const uint32_t xb[3] = {xs[0] * sz, xs[1] * sz, xs[2] * sz};
+  const uint32_t yb[3] = {ys[0] * row_b, ys[1] * row_b, ys[2] * row_b};
+  const uint32_t zb[3] = {zs[0] * plane_b, zs[1] * plane_b, zs[2] * plane_b};
 
   C h[Q];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-    const uint32_t src = (zs[cz + 1] * a.ly + ys[cy + 1]) * a.lx + xs[cx + 1];
-    h[i] = load_pop<S>(a.src, a.n, i, src, C(L::w(i)) * inv_scale);
+    h[i] = load_pop_b<S>(a.src, a.n, i, zb[cz + 1] + yb[cy + 1] + xb[cx + 1], C(L::w(i)) * inv_scale);
   });
 
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-    if (on_wall) wall_treatment<S, L>(a, h, x, y, z, cell);
+    if (on_wall) wall_treatment<S, L, FM>(a, h, x, y, z, cell);
   }
 
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
   Moments<C> m;
   if constexpr (Q == 27) m = moments_q27<C>(h);
   else m = moments_generic<L, C>(h);
-  const Macro<C> q = macro_of<C>(m, Fx, Fy, Fz);
+  const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
 
   if constexpr (MODE == kMacro) {
     if (active) {
@@ -232,7 +247,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     }
     return;
   } else {
-    C tau_eff = C(a.tau);
+    C omega;
     if constexpr (LES) {
       const bool wy = (a.walls & 2) != 0, wx = (a.walls & 1) != 0, wz = (a.walls & 4) != 0;
       // xs[0] is x+1 and xs[2] is x-1 (the upwind sources for c = -1 and +1)
@@ -252,7 +267,8 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       const C dudz = ddir<C>(u, a.n, 0, cell, czm, czp, zlo, zhi);
       const C dvdz = ddir<C>(u, a.n, 1, cell, czm, czp, zlo, zhi);
       const C dwdz = ddir<C>(u, a.n, 2, cell, czm, czp, zlo, zhi);
-      tau_eff += wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
+      const C tau_eff = C(a.tau) + wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
+      omega = C(1) - recip(tau_eff);
       if (a.write_u && active) {
         C* uo = reinterpret_cast<C*>(a.u_out);
         uo[cell] = q.ux;
@@ -260,14 +276,14 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
         uo[2 * (size_t)a.n + cell] = q.uz;
       }
     }
-    const C omega = C(1) - C(1) / tau_eff;
+    if constexpr (!LES) omega = C(a.omega);
     C out[Q];
     if constexpr (Q == 27) {
-      const NeqTensor<C> N = neq_q27<C>(m, q, Fx, Fy, Fz);
+      const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
       if constexpr (S == kF16)
-        rr_post_q27<C, false>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
+        rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
       else
-        rr_post_q27<C, true>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+        rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
     } else {
       rr_post_generic<L, C>(h, q, Fx, Fy, Fz, omega, C(1), out);
       if constexpr (S == kF16) {
@@ -287,12 +303,12 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   }
 }
 
-template <int S, class L, bool WALLS, bool LES, int MODE>
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM>
 __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
   uint32_t b = blockIdx.x;
   if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
   if (b >= a.nblocks) return;
-  fused_body<S, L, WALLS, LES, MODE>(a, b);
+  fused_body<S, L, WALLS, LES, MODE, FM>(a, b);
 }
 
 // Batched step over many periodic D3Q27 instances (ensembles): workgroup g of
@@ -303,10 +319,8 @@ template <int S>
 __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
                                                          const uint32_t* __restrict__ blk2inst,
                                                          const uint32_t* __restrict__ first_block,
-                                                         uint32_t total_blocks, uint32_t blocks_per_xcd,
-                                                         uint32_t parity) {
-  uint32_t g = blockIdx.x;
-  g = (g & 7u) * blocks_per_xcd + (g >> 3);
+                                                         uint32_t total_blocks, uint32_t parity) {
+  const uint32_t g = blockIdx.x;
   if (g >= total_blocks) return;
   const uint32_t inst = blk2inst[g];
   FusedArgs a = table[inst];
@@ -315,17 +329,16 @@ __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __rest
     a.src = a.dst;
     a.dst = const_cast<void*>(t);
   }
-  fused_body<S, D3Q27, false, false, kStep>(a, g - first_block[inst]);
+  fused_body<S, D3Q27, false, false, kStep, kForceNone>(a, g - first_block[inst]);
 }
 
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
                           uint32_t total_blocks, uint32_t parity, hipStream_t s) {
-  const uint32_t per = (total_blocks + 7) / 8;
-  const dim3 grid(8 * per), block(kBlock);
+  const dim3 grid(total_blocks), block(kBlock);
   switch (storage) {
-    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
-    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
-    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, per, parity); break;
+    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
+    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
+    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -353,38 +366,55 @@ hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double*
   return hipGetLastError();
 }
 
-template <int S, class L, bool W, bool LES>
+template <int S, class L, bool W, bool LES, int FM>
 static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   switch (mode) {
-    case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kFinish: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kFinish>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kDiag: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kDiag>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kFinish: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kFinish, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kDiag: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kDiag, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+template <int S, class L, bool W, bool LES>
+static hipError_t launch_force(int fm, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  switch (fm) {
+    case kForceNone: return launch_mode<S, L, W, LES, kForceNone>(mode, a, grid, s);
+    case kForceX: return launch_mode<S, L, W, LES, kForceX>(mode, a, grid, s);
+    default: return launch_mode<S, L, W, LES, kForceAll>(mode, a, grid, s);
+  }
+}
+
 template <int S>
-static hipError_t launch_storage(int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
+static hipError_t launch_storage(int q, bool walls, bool les, int fm, int mode, const FusedArgs& a, uint32_t grid,
                                  hipStream_t s) {
   if (q == 27) {
-    if (walls) return les ? launch_mode<S, D3Q27, true, true>(mode, a, grid, s) : launch_mode<S, D3Q27, true, false>(mode, a, grid, s);
-    return les ? launch_mode<S, D3Q27, false, true>(mode, a, grid, s) : launch_mode<S, D3Q27, false, false>(mode, a, grid, s);
+    if (walls)
+      return les ? launch_force<S, D3Q27, true, true>(fm, mode, a, grid, s)
+                 : launch_force<S, D3Q27, true, false>(fm, mode, a, grid, s);
+    return les ? launch_force<S, D3Q27, false, true>(fm, mode, a, grid, s)
+               : launch_force<S, D3Q27, false, false>(fm, mode, a, grid, s);
   }
   if (q == 19) {
     if (les) return hipErrorInvalidValue;  // WALE is provided on D3Q27 only
-    return walls ? launch_mode<S, D3Q19, true, false>(mode, a, grid, s) : launch_mode<S, D3Q19, false, false>(mode, a, grid, s);
+    // the D3Q19 path is the per-direction formulation with a general force
+    return walls ? launch_mode<S, D3Q19, true, false, kForceAll>(mode, a, grid, s)
+                 : launch_mode<S, D3Q19, false, false, kForceAll>(mode, a, grid, s);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s) {
+  const int fm = (a.fx == 0.0 && a.fy == 0.0 && a.fz == 0.0) ? kForceNone
+                 : (a.fy == 0.0 && a.fz == 0.0)             ? kForceX
+                                                            : kForceAll;
   switch (storage) {
-    case kF64: return launch_storage<kF64>(q, walls, les, mode, a, grid, s);
-    case kF32: return launch_storage<kF32>(q, walls, les, mode, a, grid, s);
-    case kF16: return launch_storage<kF16>(q, walls, les, mode, a, grid, s);
+    case kF64: return launch_storage<kF64>(q, walls, les, fm, mode, a, grid, s);
+    case kF32: return launch_storage<kF32>(q, walls, les, fm, mode, a, grid, s);
+    case kF16: return launch_storage<kF16>(q, walls, les, fm, mode, a, grid, s);
     default: return hipErrorInvalidValue;
   }
 }

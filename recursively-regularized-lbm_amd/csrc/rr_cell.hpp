@@ -113,16 +113,41 @@ struct Macro {
   C rho, dr, ux, uy, uz;
 };
 
+// Which body-force components a kernel variant carries: the channel's force is
+// along x only (macrovar.cu:70-72), Taylor-Green and cavity have none.  Terms
+// of absent components are removed at compile time.
+enum ForceMode : int { kForceNone = 0, kForceX = 1, kForceAll = 2 };
+
+// 1/x: correctly rounded in FP64; the hardware reciprocal (1 ulp) in FP32.
+RRLBM_HD RRLBM_INLINE double recip(double x) { return 1.0 / x; }
+RRLBM_HD RRLBM_INLINE float recip(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
 // macrovar (RR27V33_code_DNS_LES/macrovar.cu:17-29): rho = sum f, u = (sum c f + F/2)/rho
-template <typename C>
+template <typename C, int FM = kForceAll>
 RRLBM_HD RRLBM_INLINE Macro<C> macro_of(const Moments<C>& m, C Fx, C Fy, C Fz) {
   Macro<C> r;
   r.dr = m.dr;
   r.rho = C(1) + m.dr;
-  const C inv = C(1) / r.rho;
-  r.ux = (m.jx + C(0.5) * Fx) * inv;
-  r.uy = (m.jy + C(0.5) * Fy) * inv;
-  r.uz = (m.jz + C(0.5) * Fz) * inv;
+  const C inv = recip(r.rho);
+  if constexpr (FM == kForceNone) {
+    r.ux = m.jx * inv;
+    r.uy = m.jy * inv;
+    r.uz = m.jz * inv;
+  } else if constexpr (FM == kForceX) {
+    r.ux = (m.jx + C(0.5) * Fx) * inv;
+    r.uy = m.jy * inv;
+    r.uz = m.jz * inv;
+  } else {
+    r.ux = (m.jx + C(0.5) * Fx) * inv;
+    r.uy = (m.jy + C(0.5) * Fy) * inv;
+    r.uz = (m.jz + C(0.5) * Fz) * inv;
+  }
   return r;
 }
 
@@ -138,7 +163,7 @@ struct NeqTensor {
   C xx, yy, zz, xy, xz, yz;
 };
 
-template <typename C>
+template <typename C, int FM = kForceAll>
 RRLBM_HD RRLBM_INLINE NeqTensor<C> neq_q27(const Moments<C>& m, const Macro<C>& q, C Fx, C Fy, C Fz) {
   const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
   // sum_i H_aa f_i = 3 P_aa - (sum_i f_i - 1): the populations' own density
@@ -147,12 +172,24 @@ RRLBM_HD RRLBM_INLINE NeqTensor<C> neq_q27(const Moments<C>& m, const Macro<C>& 
   const C third = C(1.0 / 3.0) * m.dr;
   const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
   NeqTensor<C> n;
-  n.xx = m.pxx + rx * Fx * (C(1) - C(1.5) * ux * ux) - rx * ux - third;
-  n.yy = m.pyy + ry * Fy * (C(1) - C(1.5) * uy * uy) - ry * uy - third;
-  n.zz = m.pzz + rz * Fz * (C(1) - C(1.5) * uz * uz) - rz * uz - third;
-  n.xy = m.pxy + C(0.5) * (rx * Fy + ry * Fx) - rx * uy;
-  n.xz = m.pxz + C(0.5) * (rx * Fz + rz * Fx) - rx * uz;
-  n.yz = m.pyz + C(0.5) * (ry * Fz + rz * Fy) - ry * uz;
+  n.xx = m.pxx - rx * ux - third;
+  n.yy = m.pyy - ry * uy - third;
+  n.zz = m.pzz - rz * uz - third;
+  n.xy = m.pxy - rx * uy;
+  n.xz = m.pxz - rx * uz;
+  n.yz = m.pyz - ry * uz;
+  if constexpr (FM != kForceNone) {
+    n.xx += rx * Fx * (C(1) - C(1.5) * ux * ux);
+    n.xy += C(0.5) * ry * Fx;
+    n.xz += C(0.5) * rz * Fx;
+  }
+  if constexpr (FM == kForceAll) {
+    n.yy += ry * Fy * (C(1) - C(1.5) * uy * uy);
+    n.zz += rz * Fz * (C(1) - C(1.5) * uz * uz);
+    n.xy += C(0.5) * rx * Fy;
+    n.xz += C(0.5) * rx * Fz;
+    n.yz += C(0.5) * (ry * Fz + rz * Fy);
+  }
   return n;
 }
 
@@ -176,7 +213,7 @@ RRLBM_HD RRLBM_INLINE Line3<C> line3_nok2(C k0, C k1) {
 // with scale_i = w_i when per_direction_w is true (giving g_i - w_i, the shifted
 // population) and scale_i = scale otherwise (FP16 storage: (g_i - w_i) S / w_i).
 // Collision: omega = 1 - 1/tau_eff, gs = +1.  RRBC: omega = 1, gs = -1.
-template <typename C, bool per_direction_w>
+template <typename C, bool per_direction_w, int FM = kForceAll>
 RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N, C Fx, C Fy, C Fz, C omega,
                                        C gs, C scale, C (&out)[27]) {
   const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
@@ -200,43 +237,92 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
   const C Rxyz = N.xy * uz + N.xz * uy + N.yz * ux;
   const C wn = omega;
   const C n000 = C(-1.5) * (N.xx + N.yy + N.zz);
-  // force factor 1 + gs Fbar_i = b0 + phi.c
-  const C k15 = C(1.5) * gs;
-  const C phx = k15 * Fx, phy = k15 * Fy, phz = k15 * Fz;
-  const C bm1 = -(phx * ux + phy * uy + phz * uz);
-  const C b0 = C(1) + bm1;
-  const C e000 = C(1) + e000m1;
-  // coefficients of the product e*(b0 + phi.c) (reduced with c^3 = c) + omega n
-  const C K000 = bm1 + b0 * e000m1 + wn * n000;
-  const C K100 = b0 * e100 + phx * (e000 + e200) + wn * (C(-4.5) * (Ryx + Rzx));
-  const C K010 = b0 * e010 + phy * (e000 + e020) + wn * (C(-4.5) * (Rxy + Rzy));
-  const C K001 = b0 * e001 + phz * (e000 + e002) + wn * (C(-4.5) * (Rxz + Ryz));
-  const C K200 = b0 * e200 + phx * e100 + wn * (C(4.5) * N.xx);
-  const C K020 = b0 * e020 + phy * e010 + wn * (C(4.5) * N.yy);
-  const C K002 = b0 * e002 + phz * e001 + wn * (C(4.5) * N.zz);
-  const C K110 = b0 * e110 + phx * (e010 + e210) + phy * (e100 + e120) + wn * (C(9) * N.xy);
-  const C K101 = b0 * e101 + phx * (e001 + e201) + phz * (e100 + e102) + wn * (C(9) * N.xz);
-  const C K011 = b0 * e011 + phy * (e001 + e021) + phz * (e010 + e012) + wn * (C(9) * N.yz);
-  const C K210 = b0 * e210 + phx * e110 + phy * e200 + wn * (C(13.5) * Rxy);
-  const C K201 = b0 * e201 + phx * e101 + phz * e200 + wn * (C(13.5) * Rxz);
-  const C K120 = b0 * e120 + phx * e020 + phy * e110 + wn * (C(13.5) * Ryx);
-  const C K021 = b0 * e021 + phy * e011 + phz * e020 + wn * (C(13.5) * Ryz);
-  const C K102 = b0 * e102 + phx * e002 + phz * e101 + wn * (C(13.5) * Rzx);
-  const C K012 = b0 * e012 + phy * e002 + phz * e011 + wn * (C(13.5) * Rzy);
-  const C K111 = b0 * e111 + phx * e011 + phy * e101 + phz * e110 + wn * (C(27) * Rxyz);
-  const C K220 = phx * e120 + phy * e210;
-  const C K202 = phx * e102 + phz * e201;
-  const C K022 = phy * e012 + phz * e021;
-  const C K211 = phx * e111 + phy * e201 + phz * e210;
-  const C K121 = phx * e021 + phy * e111 + phz * e120;
-  const C K112 = phx * e012 + phy * e102 + phz * e111;
+  // Coefficients K of feq (1 + gs Fbar) + omega fneq, Fbar_i = 3/2 (c_i - u).F:
+  // K = b0 e + [phi.c e reduced with c^3 = c] + omega n, b0 = 1 - 3/2 gs u.F,
+  // phi = 3/2 gs F.  Products with absent force components are not formed.
+  C K000 = e000m1 + wn * n000;
+  C K100 = e100 + wn * (C(-4.5) * (Ryx + Rzx));
+  C K010 = e010 + wn * (C(-4.5) * (Rxy + Rzy));
+  C K001 = e001 + wn * (C(-4.5) * (Rxz + Ryz));
+  C K200 = e200 + wn * (C(4.5) * N.xx);
+  C K020 = e020 + wn * (C(4.5) * N.yy);
+  C K002 = e002 + wn * (C(4.5) * N.zz);
+  C K110 = e110 + wn * (C(9) * N.xy);
+  C K101 = e101 + wn * (C(9) * N.xz);
+  C K011 = e011 + wn * (C(9) * N.yz);
+  C K210 = e210 + wn * (C(13.5) * Rxy);
+  C K201 = e201 + wn * (C(13.5) * Rxz);
+  C K120 = e120 + wn * (C(13.5) * Ryx);
+  C K021 = e021 + wn * (C(13.5) * Ryz);
+  C K102 = e102 + wn * (C(13.5) * Rzx);
+  C K012 = e012 + wn * (C(13.5) * Rzy);
+  C K111 = e111 + wn * (C(27) * Rxyz);
+  C K220, K202, K022, K211, K121, K112;
+  if constexpr (FM == kForceNone) {
+    K220 = K202 = K022 = K211 = K121 = K112 = C(0);
+  } else {
+    const C k15 = C(1.5) * gs;
+    const C phx = k15 * Fx;
+    const C e000 = C(1) + e000m1;
+    C bm1 = -(phx * ux);
+    if constexpr (FM == kForceAll) bm1 -= k15 * (Fy * uy + Fz * uz);
+    // b0 e - e = bm1 e
+    K000 += bm1 + bm1 * e000m1;
+    K100 += bm1 * e100 + phx * (e000 + e200);
+    K010 += bm1 * e010;
+    K001 += bm1 * e001;
+    K200 += bm1 * e200 + phx * e100;
+    K020 += bm1 * e020;
+    K002 += bm1 * e002;
+    K110 += bm1 * e110 + phx * (e010 + e210);
+    K101 += bm1 * e101 + phx * (e001 + e201);
+    K011 += bm1 * e011;
+    K210 += bm1 * e210 + phx * e110;
+    K201 += bm1 * e201 + phx * e101;
+    K120 += bm1 * e120 + phx * e020;
+    K021 += bm1 * e021;
+    K102 += bm1 * e102 + phx * e002;
+    K012 += bm1 * e012;
+    K111 += bm1 * e111 + phx * e011;
+    K220 = phx * e120;
+    K202 = phx * e102;
+    K022 = C(0);
+    K211 = phx * e111;
+    K121 = phx * e021;
+    K112 = phx * e012;
+    if constexpr (FM == kForceAll) {
+      const C phy = k15 * Fy, phz = k15 * Fz;
+      K010 += phy * (e000 + e020);
+      K001 += phz * (e000 + e002);
+      K020 += phy * e010;
+      K002 += phz * e001;
+      K110 += phy * (e100 + e120);
+      K101 += phz * (e100 + e102);
+      K011 += phy * (e001 + e021) + phz * (e010 + e012);
+      K210 += phy * e200;
+      K201 += phz * e200;
+      K120 += phy * e110;
+      K021 += phy * e011 + phz * e020;
+      K102 += phz * e101;
+      K012 += phy * e002 + phz * e011;
+      K111 += phy * e101 + phz * e110;
+      K220 += phy * e210;
+      K202 += phz * e201;
+      K022 = phy * e012 + phz * e021;
+      K211 += phy * e201 + phz * e210;
+      K121 += phy * e111 + phz * e120;
+      K112 += phy * e102 + phz * e111;
+    }
+  }
   // tensor-product evaluation: x, then y, then z
+  constexpr bool kq = FM != kForceNone;   // K220, K202, K211, K121, K112 present
+  constexpr bool k22 = FM == kForceAll;   // K022 present
   const Line3<C> L00 = line3(K000, K100, K200);
   const Line3<C> L10 = line3(K010, K110, K210);
   const Line3<C> L01 = line3(K001, K101, K201);
-  const Line3<C> L20 = line3(K020, K120, K220);
-  const Line3<C> L02 = line3(K002, K102, K202);
-  const Line3<C> L11 = line3(K011, K111, K211);
+  const Line3<C> L20 = kq ? line3(K020, K120, K220) : line3_nok2(K020, K120);
+  const Line3<C> L02 = kq ? line3(K002, K102, K202) : line3_nok2(K002, K102);
+  const Line3<C> L11 = kq ? line3(K011, K111, K211) : line3_nok2(K011, K111);
   const Line3<C> L21 = line3_nok2(K021, K121);
   const Line3<C> L12 = line3_nok2(K012, K112);
   const C L22 = K022;
@@ -245,7 +331,7 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
     constexpr int cx = decltype(jx)::value - 1;
     const Line3<C> M0 = line3(pick(L00, cx), pick(L10, cx), pick(L20, cx));
     const Line3<C> M1 = line3(pick(L01, cx), pick(L11, cx), pick(L21, cx));
-    const Line3<C> M2 = line3(pick(L02, cx), pick(L12, cx), L22);
+    const Line3<C> M2 = k22 ? line3(pick(L02, cx), pick(L12, cx), L22) : line3_nok2(pick(L02, cx), pick(L12, cx));
     static_for<3>([&](auto jy) {
       constexpr int cy = decltype(jy)::value - 1;
       const Line3<C> V = line3(pick(M0, cy), pick(M1, cy), pick(M2, cy));

@@ -126,6 +126,7 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.wall_rr = c->p.wall_rr;
   a.fx = c->p.force[0]; a.fy = c->p.force[1]; a.fz = c->p.force[2];
   a.tau = c->p.tau; a.cw = c->p.cw;
+  a.omega = 1.0 - 1.0 / c->p.tau;
   a.lid_x = c->p.lid_u[0]; a.lid_y = c->p.lid_u[1]; a.lid_z = c->p.lid_u[2];
   a.f16_scale = c->p.f16_scale;
   a.nblocks = c->nblocks;
@@ -256,7 +257,7 @@ int rrlbm_params_default(rrlbm_params* p) {
   p->force[0] = 2.0 * u0 * u0 / p->ly;         // globals.cu:63
   p->cw = 0.5;                                 // LES.cu:12
   p->f16_scale = 4096.0;
-  p->xcd_remap = 1;
+  p->xcd_remap = 0;
   return RRLBM_OK;
 }
 

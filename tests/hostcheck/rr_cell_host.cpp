@@ -7,6 +7,13 @@
 
 using namespace rrlbm;
 
+template <typename C, int FM>
+static void fast_q27(const C (&h)[27], const Macro<C>& q, const double* F, C omega, C gs, C (&o)[27]) {
+  const Moments<C> m = moments_q27<C>(h);
+  const NeqTensor<C> N = neq_q27<C, FM>(m, q, C(F[0]), C(F[1]), C(F[2]));
+  rr_post_q27<C, true, FM>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(1), o);
+}
+
 template <class L, typename C>
 static void cell(const double* f, double rho, double ux, double uy, double uz, const double* F, double tau_eff,
                  int mode, int fast, double* out) {
@@ -17,10 +24,12 @@ static void cell(const double* f, double rho, double ux, double uy, double uz, c
   const C gs = mode == 0 ? C(1) : C(-1);
   C o[L::Q];
   if constexpr (L::Q == 27) {
-    if (fast) {
-      const Moments<C> m = moments_q27<C>(h);
-      const NeqTensor<C> N = neq_q27<C>(m, q, C(F[0]), C(F[1]), C(F[2]));
-      rr_post_q27<C, true>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(1), o);
+    if (fast == 1) {
+      fast_q27<C, kForceAll>(h, q, F, omega, gs, o);
+    } else if (fast == 2) {
+      fast_q27<C, kForceX>(h, q, F, omega, gs, o);
+    } else if (fast == 3) {
+      fast_q27<C, kForceNone>(h, q, F, omega, gs, o);
     } else {
       rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
     }
@@ -42,7 +51,8 @@ static void moments(const double* f, double* m) {
 }
 
 extern "C" {
-// q: 27 or 19; single: 1 = FP32 arithmetic; fast: 1 = moment-space D3Q27 path
+// q: 27 or 19; single: 1 = FP32 arithmetic; fast: 0 per-direction path, 1/2/3 moment-space D3Q27
+// path with all force components / force along x only / no force
 void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double ux, double uy, double uz,
                 const double* F, double tau_eff, int mode, double* out) {
   if (q == 27) {

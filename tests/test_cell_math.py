@@ -72,6 +72,20 @@ def test_cell_fp64_matches_oracle(hc, q, kind, fast, mode, consistent):
         assert np.abs(got - ref).max() < 1e-15
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("fast,force", [(2, "x"), (3, "none")])
+def test_cell_force_modes(hc, fast, force, mode):
+    """The compile-time force modes (channel: force along x only; Taylor-Green
+    and cavity: none) equal the general formulation with those forces."""
+    rng = np.random.default_rng(20 + fast)
+    for _ in range(50):
+        f, rho, u, F = random_cell(oracle.D3Q27_CUDA, rng, True)
+        F = np.array([F[0], 0.0, 0.0]) if force == "x" else np.zeros(3)
+        got = hc_cell(hc, 27, 0, fast, f, rho, u, F, 0.57, mode)
+        ref = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, 0.57, mode)
+        assert np.abs(got - ref).max() < 1e-15
+
+
 @pytest.mark.parametrize("q,kind", [(27, oracle.D3Q27_CUDA), (19, oracle.D3Q19)])
 def test_cell_fp32_arithmetic(hc, q, kind):
     rng = np.random.default_rng(3)
