@@ -102,25 +102,10 @@ def cpu_baseline(steps=2):
                       % (steps, LX, LY, LZ, dt)}
 
 
-def ensemble_instances(seed=5, count=64):
-    """64 periodic 3D Taylor-Green boxes: N in {32,48,64,96,128,192}, Ma in
-    U[0.01, 0.1] (u0 = Ma cs), Re in U[100, 1600], nu = u0 (N / 2 pi) / Re."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        n = int(rng.choice([32, 48, 64, 96, 128, 192]))
-        ma = rng.uniform(0.01, 0.1)
-        re = rng.uniform(100.0, 1600.0)
-        u0 = ma / np.sqrt(3.0)
-        nu = u0 * (n / (2 * np.pi)) / re
-        out.append((n, n, n, 3.0 * nu + 0.5, u0))
-    return out
-
-
 def run_ensemble(rrlbm, steps, warmup, storage=1):
-    inst = ensemble_instances()
+    from rrlbm import configs
+
+    inst = configs.ensemble_instances()
     res = {}
     with rrlbm.Batch(inst, storage=storage, keep_reference=False) as b:
         cells = b.cells()
@@ -159,6 +144,178 @@ def main_ensemble(args):
     print(json.dumps(line))
 
 
+def timed(s, steps, warmup):
+    """(MLUPS over the wall clock, mean kernel ms per step) of `steps` fused steps."""
+    s.fused_pack()
+    s.fused_step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ms = s.fused_step_timed(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    return s.n * steps / wall / 1e6, ms / steps
+
+
+def line_for(metric, workload, mlups, ms, blu, dtype, steps, warmup, extra):
+    ach = blu * mlups * 1e6 / 1e9
+    d = {"metric": metric, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
+         "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+         "data": "synthetic", "config": {"workload": workload},
+         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                      "frac": round(ach / PEAK_GBS, 4), "traffic": None, "bytes_per_lu": blu}}
+    d.update(extra)
+    return d
+
+
+def tg_cpu_baseline(steps=2000):
+    """The FP64 oracle (C/OpenMP) over the full C1 run: 2 000 steps of 100x100x4."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import math
+
+    import oracle
+
+    u0 = 0.01 / math.sqrt(3.0)
+    nu = u0 * 100 / 100.0
+    g = oracle.Fields(oracle.D3Q27_CUDA, 100, 100, 4)
+    g.init_tg_fields(u0)
+    g.init_equilibrium()
+    g.set_force((0.0, 0.0, 0.0))
+    t0 = time.perf_counter()
+    g.step(steps, 3 * nu + 0.5, walls=(0, 0, 0), wall_rr=0)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(40000 * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": "%d steps of 100x100x4 (FP64 oracle, C/OpenMP), %.1f s" % (steps, dt)}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main_configs(args):
+    """C1 Taylor-Green, C2 cavity, C4 LES channel: one JSON line each."""
+    import rrlbm
+    from rrlbm import configs
+
+    rrlbm.load()
+    out = []
+    # C4: channel LES-WALE 192x64x96, lagged gradient (one launch per step) and reference order
+    for st in (2, 1, 0):
+        for lagged in (1, 0):
+            s, meta = configs.channel_les(storage=st, les_lagged=lagged)
+            mlups, ms = timed(s, args.steps, args.warmup)
+            s.close()
+            log("C4 storage %d lagged %d: %.1f MLUPS" % (st, lagged, mlups))
+            blu = 2 * 27 * STORAGE_BYTES[st] + 6 * (8 if st == 0 else 4) + (0 if lagged else 27 * STORAGE_BYTES[st])
+            out.append(line_for("MLUPS, channel LES-WALE Re_tau=180 192x64x96 D3Q27", "C4 LES %s %s" % (
+                STORAGE_NAME[st], "lagged" if lagged else "reference-order"), mlups, ms, blu, STORAGE_NAME[st],
+                args.steps, args.warmup, {}))
+    # C2: cavity 128^3 Re 1000, D3Q19 and D3Q27
+    for q in (19, 27):
+        for st in (1, 0):
+            s, meta = configs.cavity(storage=st, npop=q)
+            mlups, ms = timed(s, args.steps, args.warmup)
+            s.close()
+            log("C2 D3Q%d storage %d: %.1f MLUPS" % (q, st, mlups))
+            out.append(line_for("MLUPS, lid-driven cavity 128^3 Re=1000", "C2 cavity D3Q%d %s" % (q, STORAGE_NAME[st]),
+                                mlups, ms, 2 * q * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup, {}))
+    # C1: Taylor-Green 100x100x4 (40 000 cells: launch-bound), full 2 000 steps;
+    # the CPU oracle runs the same 2 000 reference-order steps
+    cpu = tg_cpu_baseline()
+    for st in (0, 1):
+        s, meta = configs.taylor_green_2d(storage=st)
+        mlups, ms = timed(s, 2000, 20)
+        s.close()
+        log("C1 storage %d: %.1f MLUPS" % (st, mlups))
+        out.append(line_for("MLUPS, Taylor-Green 100x100x4", "C1 TG %s" % STORAGE_NAME[st], mlups, ms,
+                            2 * 27 * STORAGE_BYTES[st], STORAGE_NAME[st], 2000, 20, {"cpu_baseline": cpu}))
+    for d in out:
+        print(json.dumps(d))
+
+
+def main_validate(args):
+    """Precision checks over long runs: FP32 / FP16 storage against the FP64
+    run of the same kernels (and a second-seed FP64 run as the chaotic floor)."""
+    import numpy as np
+
+    import rrlbm
+    from rrlbm import configs
+
+    rrlbm.load()
+    res = {}
+    # C1: Taylor-Green energy decay after 2000 steps
+    ke = {}
+    for st in (0, 1, 2):
+        s, meta = configs.taylor_green_2d(storage=st)
+        k0 = s.diag_flow()["kinetic_energy"]
+        s.fused_pack()
+        s.fused_step(1999)
+        ke[st] = s.fused_diag()["kinetic_energy"] / k0
+        s.close()
+    log("C1 done")
+    analytic = float(np.exp(-4 * meta["k"] ** 2 * meta["nu"] * 2000))
+    orc = np.loadtxt(os.path.join(REPO, "tests", "golden", "tg2d_c1_ke_oracle.txt"))
+    orc_ratio = float(orc[-1, 1] / orc[0, 1])
+    res["C1_taylor_green"] = {"ke_ratio_f64": ke[0], "ke_ratio_oracle": orc_ratio, "analytic": analytic,
+                              "rel_err_f64_vs_oracle": abs(ke[0] / orc_ratio - 1),
+                              "rel_err_f64_vs_analytic": abs(ke[0] / analytic - 1),
+                              "rel_err_f32_vs_f64": abs(ke[1] / ke[0] - 1), "rel_err_f16_vs_f64": abs(ke[2] / ke[0] - 1)}
+    # C2: cavity 128^3 centre lines after 20 000 steps, FP32 vs FP64
+    for q in (19, 27):
+        lines = {}
+        for st in (0, 1):
+            s, meta = configs.cavity(storage=st, npop=q)
+            s.fused_pack()
+            s.fused_step(args.cavity_steps - 1)
+            s.fused_unpack()
+            n = s.lx
+            U = s.download(rrlbm.Field.UX).reshape(n, n, n)
+            V = s.download(rrlbm.Field.UY).reshape(n, n, n)
+            c = n // 2
+            lines[st] = (0.25 * (U[c - 1, :, c - 1] + U[c - 1, :, c] + U[c, :, c - 1] + U[c, :, c]),
+                         0.25 * (V[c - 1, c - 1, :] + V[c - 1, c, :] + V[c, c - 1, :] + V[c, c, :]))
+            s.close()
+            log("C2 D3Q%d storage %d done" % (q, st))
+        res["C2_cavity_D3Q%d" % q] = {"steps": args.cavity_steps,
+                                      "max_abs_du_over_lid": float(np.abs(lines[1][0] - lines[0][0]).max() / 0.1),
+                                      "max_abs_dv_over_lid": float(np.abs(lines[1][1] - lines[0][1]).max() / 0.1),
+                                      "u_min_f64": float(lines[0][0].min())}
+
+    # C3 / C4: channel mean U+ profiles
+    def channel_stats(make, st, seed, steps, every, window):
+        s, meta = make(storage=st, seed=seed)
+        s.fused_pack()
+        acc, cnt, done = None, 0, 1
+        while done < steps:
+            s.fused_step(min(every, steps - done))
+            done = min(done + every, steps)
+            if done > steps - window:
+                s.fused_unpack()
+                p = s.output_profile()
+                acc = p.copy() if acc is None else acc + p
+                cnt += 1
+        d = s.fused_diag()
+        s.close()
+        log("channel %dx%dx%d storage %d seed %d: %d steps" % (s.lx, s.ly, s.lz, st, seed, steps))
+        prof = acc / cnt
+        return prof[:, 0] / meta["u_tau"], np.sqrt(np.maximum(prof[:, 3], 0)) / meta["u_tau"], d["u_sum"][0] / d[
+            "ncells"] / meta["u_tau"]
+
+    for name, make, steps, window in (("C4_channel_les", configs.channel_les, args.les_steps, 20000),
+                                      ("C3_channel_dns", configs.channel, args.dns_steps, 10000)):
+        base = channel_stats(make, 0, 2024, steps, 1000, window)
+        other = channel_stats(make, 0, 7, steps, 1000, window)
+        r = {"steps": steps, "window": window, "ubulk_plus_f64": base[2],
+             "umean_plus_f64_seedB_rel": float(np.abs(other[0] - base[0]).max() / base[0].max())}
+        for st in (1, 2):
+            o = channel_stats(make, st, 2024, steps, 1000, window)
+            r["umean_plus_%s_rel" % STORAGE_NAME[st]] = float(np.abs(o[0] - base[0]).max() / base[0].max())
+            r["urms_plus_%s_abs" % STORAGE_NAME[st]] = float(np.abs(o[1] - base[1]).max())
+            r["ubulk_plus_%s" % STORAGE_NAME[st]] = o[2]
+        res[name] = r
+    print(json.dumps({"workload": "validate", "results": res}))
+
+
 def main_copy():
     """Calibration workload for rocprofv3's FETCH_SIZE / WRITE_SIZE counters: a
     plain device-to-device copy of 1 GiB, five times."""
@@ -180,7 +337,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "copy"])
+    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "configs", "validate", "copy"])
+    ap.add_argument("--cavity-steps", type=int, default=20000)
+    ap.add_argument("--les-steps", type=int, default=50000)
+    ap.add_argument("--dns-steps", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -189,6 +349,10 @@ def main():
         return main_ensemble(args)
     if args.workload == "copy":
         return main_copy()
+    if args.workload == "configs":
+        return main_configs(args)
+    if args.workload == "validate":
+        return main_validate(args)
     import rrlbm
 
     rrlbm.load()

@@ -88,7 +88,9 @@ typedef struct rrlbm_params {
     double f16_scale;           /* FP16 storage: (f_i - w_i) * f16_scale / w_i is stored;
                                    0 selects 4096 */
     int32_t xcd_remap;          /* 1: workgroup order remapped per XCD; 0: linear (default, faster on MI355X) */
-    int32_t reserved[7];
+    int32_t cells_per_thread;   /* fused D3Q27 step without LES: 1 or 2 cells per thread
+                                   (64 cells apart); 0 = the tuned / default choice */
+    int32_t reserved[6];
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;

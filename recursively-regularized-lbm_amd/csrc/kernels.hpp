@@ -86,6 +86,8 @@ struct FusedArgs {
 // Host launchers (kernels_fused.hip)
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s);
+// Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
+hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
                           uint32_t total_blocks, uint32_t parity, hipStream_t s);
 hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double* out, hipStream_t s);

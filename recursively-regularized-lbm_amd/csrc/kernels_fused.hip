@@ -311,6 +311,73 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
   fused_body<S, L, WALLS, LES, MODE, FM>(a, b);
 }
 
+// Two cells per thread (STEP mode, D3Q27, no LES): lane l of wavefront w of
+// workgroup b takes cells base + l and base + l + 64, base = b*512 + w*128, so
+// every load and store instruction still covers 64 consecutive cells while
+// each wavefront keeps twice as many population loads in flight -- the better
+// layout when the per-cell arithmetic is a large share of the time.
+template <int S, bool WALLS, int FM>
+__global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
+  using C = typename StoreT<S>::C;
+  using L = D3Q27;
+  constexpr int Q = 27;
+  uint32_t b = blockIdx.x;
+  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  if (b >= a.nblocks) return;
+  const uint32_t base = b * (2 * kBlock) + (threadIdx.x >> 6) * 128 + (threadIdx.x & 63);
+  const uint32_t craw[2] = {base, base + 64};
+  const uint32_t lxly = a.lx * a.ly;
+  constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
+  const uint32_t row_b = a.lx * sz, plane_b = lxly * sz;
+  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  C h[2][Q];
+  uint32_t cell[2], cx_[2], cy_[2], cz_[2];
+  bool active[2];
+  static_for<2>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    active[k] = craw[k] < a.n;
+    cell[k] = active[k] ? craw[k] : a.n - 1;
+    const uint32_t z = fastdiv(cell[k], a.div_lxly);
+    const uint32_t rem = cell[k] - z * lxly;
+    const uint32_t y = fastdiv(rem, a.div_lx);
+    const uint32_t x = rem - y * a.lx;
+    cx_[k] = x; cy_[k] = y; cz_[k] = z;
+    const uint32_t xb[3] = {(x + 1 == a.lx ? 0u : x + 1) * sz, x * sz, (x == 0 ? a.lx - 1 : x - 1) * sz};
+    const uint32_t yb[3] = {(y + 1 == a.ly ? 0u : y + 1) * row_b, y * row_b, (y == 0 ? a.ly - 1 : y - 1) * row_b};
+    const uint32_t zb[3] = {(z + 1 == a.lz ? 0u : z + 1) * plane_b, z * plane_b, (z == 0 ? a.lz - 1 : z - 1) * plane_b};
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      h[k][i] = load_pop_b<S>(a.src, a.n, i, zb[L::cz(i) + 1] + yb[L::cy(i) + 1] + xb[L::cx(i) + 1],
+                              C(L::w(i)) * inv_scale);
+    });
+  });
+  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  const C omega = C(a.omega);
+  static_for<2>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if constexpr (WALLS) {
+      const uint32_t x = cx_[k], y = cy_[k], z = cz_[k];
+      const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) ||
+                           ((a.walls & 2) && (y == 0 || y == a.ly - 1)) || ((a.walls & 4) && (z == 0 || z == a.lz - 1));
+      if (on_wall) wall_treatment<S, L, FM>(a, h[k], x, y, z, cell[k]);
+    }
+    const Moments<C> m = moments_q27<C>(h[k]);
+    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
+    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+    C out[Q];
+    if constexpr (S == kF16)
+      rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
+    else
+      rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+    if (active[k]) {
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        store_pop<S>(a.dst, a.n, i, cell[k], out[i]);
+      });
+    }
+  });
+}
+
 // Batched step over many periodic D3Q27 instances (ensembles): workgroup g of
 // the launch belongs to instance blk2inst[g]; each instance's domain,
 // relaxation time and A/B buffers come from its FusedArgs in `table`
@@ -406,11 +473,33 @@ static hipError_t launch_storage(int q, bool walls, bool les, int fm, int mode, 
   return hipErrorInvalidValue;
 }
 
+template <int S, bool W>
+static hipError_t launch_pair_force(int fm, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  switch (fm) {
+    case kForceNone: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceNone>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kForceX: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceX>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceAll>), dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+static int force_mode_of(const FusedArgs& a) {
+  return (a.fx == 0.0 && a.fy == 0.0 && a.fz == 0.0) ? kForceNone : (a.fy == 0.0 && a.fz == 0.0) ? kForceX : kForceAll;
+}
+
+hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  const int fm = force_mode_of(a);
+  switch (storage) {
+    case kF64: return walls ? launch_pair_force<kF64, true>(fm, a, grid, s) : launch_pair_force<kF64, false>(fm, a, grid, s);
+    case kF32: return walls ? launch_pair_force<kF32, true>(fm, a, grid, s) : launch_pair_force<kF32, false>(fm, a, grid, s);
+    case kF16: return walls ? launch_pair_force<kF16, true>(fm, a, grid, s) : launch_pair_force<kF16, false>(fm, a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s) {
-  const int fm = (a.fx == 0.0 && a.fy == 0.0 && a.fz == 0.0) ? kForceNone
-                 : (a.fy == 0.0 && a.fz == 0.0)             ? kForceX
-                                                            : kForceAll;
+  const int fm = force_mode_of(a);
   switch (storage) {
     case kF64: return launch_storage<kF64>(q, walls, les, fm, mode, a, grid, s);
     case kF32: return launch_storage<kF32>(q, walls, les, fm, mode, a, grid, s);

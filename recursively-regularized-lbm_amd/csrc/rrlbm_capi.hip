@@ -146,6 +146,26 @@ int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
   return RRLBM_OK;
 }
 
+// Two cells per thread for the plain D3Q27 step only when requested: on
+// MI355X one cell per thread is faster for every storage (higher occupancy).
+bool use_pair(const rrlbm_ctx* c) {
+  if (c->q != 27 || c->p.use_les) return false;
+  return c->p.cells_per_thread == 2;
+}
+
+int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
+  if (use_pair(c)) {
+    FusedArgs p = a;
+    p.nblocks = (c->n + 2 * kBlock - 1) / (2 * kBlock);
+    p.blocks_per_xcd = (p.nblocks + 7) / 8;
+    const uint32_t grid = c->p.xcd_remap ? 8u * p.blocks_per_xcd : p.nblocks;
+    hipError_t e = launch_fused_pair(c->p.storage, has_walls(c->p), p, grid, c->stream);
+    if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
+    return RRLBM_OK;
+  }
+  return launch(c, kStep, a);
+}
+
 int fused_steps(rrlbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     FusedArgs a = fused_args(c);
@@ -163,7 +183,7 @@ int fused_steps(rrlbm_ctx* c, int nsteps) {
         a.write_u = 0;
       }
     }
-    int rc = launch(c, kStep, a);
+    int rc = c->p.use_les ? launch(c, kStep, a) : launch_step(c, a);
     if (rc) return rc;
     std::swap(c->pop_a, c->pop_b);
     if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);

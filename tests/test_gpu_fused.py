@@ -100,3 +100,21 @@ def test_pack_state_error_before_pack(rr):
         with pytest.raises(rr.RRLBMError) as ei:
             s.fused_step(1)
         assert ei.value.status == 3
+
+
+@pytest.mark.parametrize("walls", [(0, 1, 0), (1, 1, 0), (0, 0, 0)])
+def test_two_cells_per_thread_matches_reference_order(rr, walls):
+    """The pair kernel (two x-adjacent cells per thread) gives the reference
+    order's result in FP64 and agrees with the one-cell kernel in FP32/FP16."""
+    g = random_state(oracle.D3Q27_CUDA, 12, 10, 6, seed=31, force=(3e-5, 0.0, 0.0))
+    tau = 0.54
+    kw = dict(tau=tau, force=(3e-5, 0, 0), wall_x=walls[0], wall_y=walls[1], wall_z=walls[2],
+              wall_rr=1 if any(walls) else 0)
+    got_f, got_m, _ = run_fused(rr, g, 9, storage=0, cells_per_thread=2, **kw)
+    ref = g.copy()
+    ref.step(9, tau, walls=walls, wall_rr=kw["wall_rr"])
+    assert maxabs(got_f, ref.f) < 1e-12
+    for st, tol in ((1, 1e-6), (2, 1e-3)):
+        f1, _, _ = run_fused(rr, g, 9, storage=st, cells_per_thread=1, **kw)
+        f2, _, _ = run_fused(rr, g, 9, storage=st, cells_per_thread=2, **kw)
+        assert maxabs(f1, f2) < tol
