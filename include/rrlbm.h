@@ -52,6 +52,13 @@ typedef enum rrlbm_status {
 /* storage of the fused path's populations */
 typedef enum rrlbm_storage { RRLBM_STORE_F64 = 0, RRLBM_STORE_F32 = 1, RRLBM_STORE_F16 = 2 } rrlbm_storage;
 
+/* rounding of the FP16 store.  Round-to-nearest loses every per-step change
+ * smaller than half an FP16 step (a slowly evolving flow stalls: a laminar
+ * channel stops accelerating under its body force); stochastic rounding keeps
+ * such changes in expectation.  The dither is a fixed function of (cell,
+ * population, step, instance), so runs are reproducible. */
+typedef enum rrlbm_f16_rounding { RRLBM_F16_STOCHASTIC = 0, RRLBM_F16_NEAREST = 1 } rrlbm_f16_rounding;
+
 /* reference-layout FP64 fields owned by the context (RR27V33_code_DNS_LES/globals.cu:67-71) */
 typedef enum rrlbm_field {
     RRLBM_FIELD_F = 0,      /* d_f, NPOP*N */
@@ -90,7 +97,8 @@ typedef struct rrlbm_params {
     int32_t xcd_remap;          /* 1: workgroup order remapped per XCD; 0: linear (default, faster on MI355X) */
     int32_t cells_per_thread;   /* fused D3Q27 step without LES: 1 or 2 cells per thread
                                    (64 cells apart); 0 = the tuned / default choice */
-    int32_t reserved[6];
+    int32_t f16_rounding;       /* rrlbm_f16_rounding of the FP16 store (default stochastic) */
+    int32_t reserved[5];
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;
@@ -239,7 +247,9 @@ int rrlbm_batch_init_taylor_green_3d(rrlbm_batch* b);
 int rrlbm_batch_step(rrlbm_batch* b, int nsteps);
 /* nsteps, one launch per instance per step, in order on one stream. */
 int rrlbm_batch_step_sequential(rrlbm_batch* b, int nsteps);
-/* Capture nsteps batched launches into a HIP graph (replaces a previous one). */
+/* Capture nsteps batched launches into a HIP graph (replaces a previous one).
+ * With FP16 storage every replay repeats the rounding dither of the captured
+ * steps (the step counter is a launch argument fixed at capture). */
 int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps);
 /* Launch the captured graph `times` times (times * nsteps steps). */
 int rrlbm_batch_graph_launch(rrlbm_batch* b, int times);

@@ -78,6 +78,9 @@ struct FusedArgs {
   double omega;         // 1 - 1/tau (without LES)
   double lid_x, lid_y, lid_z;
   double f16_scale;     // FP16: stored value = (f_i - w_i) * f16_scale / w_i
+  uint32_t f16_round;   // FP16 store: 0 stochastic, 1 nearest
+  uint32_t sr_step;     // step counter of the stochastic-rounding dither
+  uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)
   uint32_t nblocks;     // workgroups covering the domain
   uint32_t blocks_per_xcd;
   uint32_t xcd_remap;   // nonzero: launch is padded to 8*blocks_per_xcd and remapped per XCD
@@ -89,7 +92,10 @@ hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, cons
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
-                          uint32_t total_blocks, uint32_t parity, hipStream_t s);
+                          uint32_t total_blocks, uint32_t parity, uint32_t step, hipStream_t s);
+// Dither offset of batch instance k (the FP16 stochastic rounding of instance
+// k then differs from that of the other instances at the same cell index).
+inline uint32_t instance_salt(uint32_t k) { return k * 0x68E31DA5u; }
 hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double* out, hipStream_t s);
 
 }  // namespace rrlbm

@@ -46,12 +46,61 @@ RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_
 
 // Post-collision populations are written once and read by the next launch:
 // non-temporal stores keep them from displacing the gather's cache lines.
-template <int S>
-RRLBM_INLINE __device__ void store_pop(void* base, uint32_t n, int i, uint32_t cell, typename StoreT<S>::C v) {
-  using T = typename StoreT<S>::T;
+template <class T>
+RRLBM_INLINE __device__ void store_raw(void* base, uint32_t n, int i, uint32_t cell, T v) {
   char* p = reinterpret_cast<char*>(base) + (size_t)i * n * sizeof(T);
-  const uint32_t off = cell * (uint32_t)sizeof(T);
-  __builtin_nontemporal_store(T(v), reinterpret_cast<T*>(p + off));
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p + cell * (uint32_t)sizeof(T)));
+}
+
+// Stochastic rounding of an FP32 value to FP16: the 13 dither bits (top bits
+// of r) are added to the 13 low mantissa bits that FP16 drops and the sum is
+// truncated.  The result is the upper FP16 neighbour with probability
+// (dropped part) / (FP16 step), so the stored value equals v in expectation;
+// the truncated value is exact in FP16 (normal range), so the conversion adds
+// no second rounding.  A carry into the exponent gives the next binade's
+// first value, as it should.
+RRLBM_INLINE __device__ _Float16 f16_stochastic(float v, uint32_t r) {
+  const uint32_t b = (__float_as_uint(v) + (r >> 19)) & 0xFFFFE000u;
+  return (_Float16)__uint_as_float(b);
+}
+
+// 32-bit integer mixer ("lowbias32", from C. Wellons' hash-prospector search).
+RRLBM_INLINE __device__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Dither of one cell at one step, hashed from both, so that a cell's dither
+// at the next step is independent of this one.  (A dither that advances by a
+// fixed increment per step -- a Weyl sequence -- correlates with the value it
+// rounds: the next value depends on this step's rounding, and the mean flow
+// drifts.)  Population i adds i times an odd constant.
+RRLBM_INLINE __device__ uint32_t dither_base(uint32_t cell, uint32_t step, uint32_t salt) {
+  return mix32(mix32(cell + salt) + step * 0x9E3779B9u);
+}
+
+// Store the Q post-collision values of one cell (FP16: scaled values).
+template <int S, int Q>
+RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, uint32_t cell, const typename StoreT<S>::C (&out)[Q]) {
+  using T = typename StoreT<S>::T;
+  if constexpr (S == kF16) {
+    if (a.f16_round == 0) {
+      const uint32_t r0 = dither_base(cell, a.sr_step, a.sr_salt);
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        store_raw<T>(a.dst, a.n, i, cell, f16_stochastic(out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
+      });
+      return;
+    }
+  }
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    store_raw<T>(a.dst, a.n, i, cell, T(out[i]));
+  });
 }
 
 // Does population i (velocity x,y,z) enter the domain through face f?
@@ -294,12 +343,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
         });
       }
     }
-    if (active) {
-      static_for<Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        store_pop<S>(a.dst, a.n, i, cell, out[i]);
-      });
-    }
+    if (active) store_cell<S, Q>(a, cell, out);
   }
 }
 
@@ -369,28 +413,25 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
       rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
     else
       rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
-    if (active[k]) {
-      static_for<Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        store_pop<S>(a.dst, a.n, i, cell[k], out[i]);
-      });
-    }
+    if (active[k]) store_cell<S, Q>(a, cell[k], out);
   });
 }
 
 // Batched step over many periodic D3Q27 instances (ensembles): workgroup g of
 // the launch belongs to instance blk2inst[g]; each instance's domain,
 // relaxation time and A/B buffers come from its FusedArgs in `table`
-// (parity 1 swaps src and dst, so one captured graph alternates buffers).
+// (parity 1 swaps src and dst, so one captured graph alternates buffers;
+// `step` drives the FP16 stochastic-rounding dither).
 template <int S>
 __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
                                                          const uint32_t* __restrict__ blk2inst,
                                                          const uint32_t* __restrict__ first_block,
-                                                         uint32_t total_blocks, uint32_t parity) {
+                                                         uint32_t total_blocks, uint32_t parity, uint32_t step) {
   const uint32_t g = blockIdx.x;
   if (g >= total_blocks) return;
   const uint32_t inst = blk2inst[g];
   FusedArgs a = table[inst];
+  a.sr_step = step;
   if (parity) {
     const void* t = a.src;
     a.src = a.dst;
@@ -400,12 +441,12 @@ __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __rest
 }
 
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
-                          uint32_t total_blocks, uint32_t parity, hipStream_t s) {
+                          uint32_t total_blocks, uint32_t parity, uint32_t step, hipStream_t s) {
   const dim3 grid(total_blocks), block(kBlock);
   switch (storage) {
-    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
-    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
-    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity); break;
+    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
+    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
+    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -40,6 +40,7 @@ struct rrlbm_ctx {
   double* profiles = nullptr;
   uint32_t nblocks = 0;
   bool packed = false;
+  uint32_t sr_step = 0;  // fused steps taken since creation (stochastic-rounding dither)
   bool ref_ok = true;  // reference-layout fields allocated
   size_t bytes = 0;
   std::string err;
@@ -129,6 +130,8 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.omega = 1.0 - 1.0 / c->p.tau;
   a.lid_x = c->p.lid_u[0]; a.lid_y = c->p.lid_u[1]; a.lid_z = c->p.lid_u[2];
   a.f16_scale = c->p.f16_scale;
+  a.f16_round = (uint32_t)c->p.f16_rounding;
+  a.sr_step = c->sr_step;
   a.nblocks = c->nblocks;
   a.xcd_remap = c->p.xcd_remap ? 1u : 0u;
   a.blocks_per_xcd = (c->nblocks + 7) / 8;
@@ -185,6 +188,7 @@ int fused_steps(rrlbm_ctx* c, int nsteps) {
     }
     int rc = c->p.use_les ? launch(c, kStep, a) : launch_step(c, a);
     if (rc) return rc;
+    ++c->sr_step;
     std::swap(c->pop_a, c->pop_b);
     if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
   }
@@ -326,6 +330,8 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     if (!walled) return fail(nullptr, RRLBM_E_INVALID, "lid_face %d lies on a periodic axis", p.lid_face);
   }
   if (p.f16_scale == 0.0) p.f16_scale = 4096.0;
+  if (p.f16_rounding != RRLBM_F16_STOCHASTIC && p.f16_rounding != RRLBM_F16_NEAREST)
+    return fail(nullptr, RRLBM_E_INVALID, "f16_rounding must be 0 (stochastic) or 1 (nearest)");
   if (!(p.f16_scale > 0.0)) return fail(nullptr, RRLBM_E_INVALID, "f16_scale must be positive");
   const uint64_t per_pop = n64 * storage_size(p.storage);
   if (per_pop >= (1ull << 32)) return fail(nullptr, RRLBM_E_INVALID, "one population array exceeds 4 GiB");
@@ -712,6 +718,7 @@ struct rrlbm_batch {
   uint32_t* d_first = nullptr;
   uint32_t total_blocks = 0;
   uint32_t parity = 0;  // 1: every instance's current state sits in its table dst buffer
+  uint32_t sr_step = 0;  // eager batched / sequential steps taken (FP16 rounding dither)
   bool ready = false;   // instances initialised and packed, table uploaded
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -760,6 +767,7 @@ int batch_upload_table(rrlbm_batch* b) {
   std::vector<FusedArgs> t(b->inst.size());
   for (size_t k = 0; k < b->inst.size(); ++k) {
     t[k] = fused_args(b->inst[k]);
+    t[k].sr_salt = instance_salt((uint32_t)k);
     if (b->parity) {
       const void* cur = t[k].src;
       t[k].src = t[k].dst;
@@ -772,18 +780,25 @@ int batch_upload_table(rrlbm_batch* b) {
 }
 
 int batch_launch(rrlbm_batch* b) {
-  hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, b->parity, b->stream);
+  hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, b->parity,
+                                b->sr_step, b->stream);
   if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "batched launch failed: %s", hipGetErrorString(e));
+  ++b->sr_step;
   batch_swap_all(b);
   return RRLBM_OK;
 }
 
 int batch_sequential(rrlbm_batch* b, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
-    for (rrlbm_ctx* c : b->inst) {
-      hipError_t e = launch_fused(c->p.storage, 27, false, false, kStep, fused_args(c), fused_grid(c), b->stream);
+    for (size_t k = 0; k < b->inst.size(); ++k) {
+      rrlbm_ctx* c = b->inst[k];
+      FusedArgs a = fused_args(c);
+      a.sr_step = b->sr_step;
+      a.sr_salt = instance_salt((uint32_t)k);
+      hipError_t e = launch_fused(c->p.storage, 27, false, false, kStep, a, fused_grid(c), b->stream);
       if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "instance launch failed: %s", hipGetErrorString(e));
     }
+    ++b->sr_step;
     batch_swap_all(b);
   }
   return RRLBM_OK;
@@ -936,7 +951,8 @@ int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps) {
   BCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   uint32_t par = start;
   for (int s = 0; s < nsteps; ++s) {
-    hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, par, b->stream);
+    hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, par,
+                                  b->sr_step + (uint32_t)s, b->stream);
     if (e != hipSuccess) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(b->stream, &g);

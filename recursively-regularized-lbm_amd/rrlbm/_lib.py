@@ -29,7 +29,8 @@ class Params(ctypes.Structure):
         ("wall_rr", ctypes.c_int32), ("lid_face", ctypes.c_int32),
         ("tau", ctypes.c_double), ("force", ctypes.c_double * 3), ("cw", ctypes.c_double),
         ("lid_u", ctypes.c_double * 3), ("f16_scale", ctypes.c_double),
-        ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("f16_rounding", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 

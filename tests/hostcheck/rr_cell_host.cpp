@@ -14,6 +14,16 @@ static void fast_q27(const C (&h)[27], const Macro<C>& q, const double* F, C ome
   rr_post_q27<C, true, FM>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(1), o);
 }
 
+// The FP16-storage form: the polynomial yields (f_i - w_i) * scale / w_i.
+template <typename C, int FM>
+static void scaled_q27(const C (&h)[27], const Macro<C>& q, const double* F, C omega, C gs, C (&o)[27]) {
+  constexpr double kScale = 4096.0;
+  const Moments<C> m = moments_q27<C>(h);
+  const NeqTensor<C> N = neq_q27<C, FM>(m, q, C(F[0]), C(F[1]), C(F[2]));
+  rr_post_q27<C, false, FM>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(kScale), o);
+  for (int i = 0; i < 27; ++i) o[i] = C(double(o[i]) * D3Q27::w(i) / kScale);
+}
+
 template <class L, typename C>
 static void cell(const double* f, double rho, double ux, double uy, double uz, const double* F, double tau_eff,
                  int mode, int fast, double* out) {
@@ -30,6 +40,12 @@ static void cell(const double* f, double rho, double ux, double uy, double uz, c
       fast_q27<C, kForceX>(h, q, F, omega, gs, o);
     } else if (fast == 3) {
       fast_q27<C, kForceNone>(h, q, F, omega, gs, o);
+    } else if (fast == 4) {
+      scaled_q27<C, kForceAll>(h, q, F, omega, gs, o);
+    } else if (fast == 5) {
+      scaled_q27<C, kForceX>(h, q, F, omega, gs, o);
+    } else if (fast == 6) {
+      scaled_q27<C, kForceNone>(h, q, F, omega, gs, o);
     } else {
       rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
     }
@@ -52,7 +68,8 @@ static void moments(const double* f, double* m) {
 
 extern "C" {
 // q: 27 or 19; single: 1 = FP32 arithmetic; fast: 0 per-direction path, 1/2/3 moment-space D3Q27
-// path with all force components / force along x only / no force
+// path with all force components / force along x only / no force, 4/5/6 the same in the
+// FP16-storage form (values scaled by 4096 / w_i)
 void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double ux, double uy, double uz,
                 const double* F, double tau_eff, int mode, double* out) {
   if (q == 27) {

@@ -60,7 +60,7 @@ def test_invalid_parameters_are_reported_not_fatal(rr):
     lib = rr.load()
     h = ctypes.c_void_p()
     for bad in (dict(npop=15), dict(tau=0.4), dict(use_les=1, npop=19), dict(storage=7), dict(lx=0),
-                dict(lid_face=3, wall_y=0)):
+                dict(lid_face=3, wall_y=0), dict(f16_rounding=2)):
         p = rr.default_params(**bad)
         st = lib.rrlbm_create(ctypes.byref(p), ctypes.byref(h))
         assert st in (1, 6)

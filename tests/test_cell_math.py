@@ -73,14 +73,18 @@ def test_cell_fp64_matches_oracle(hc, q, kind, fast, mode, consistent):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("fast,force", [(2, "x"), (3, "none")])
+@pytest.mark.parametrize("fast,force", [(2, "x"), (3, "none"), (4, "all"), (5, "x"), (6, "none")])
 def test_cell_force_modes(hc, fast, force, mode):
     """The compile-time force modes (channel: force along x only; Taylor-Green
-    and cavity: none) equal the general formulation with those forces."""
+    and cavity: none) equal the general formulation with those forces; modes
+    4-6 are the FP16-storage form of the polynomial (output scaled by S/w_i)."""
     rng = np.random.default_rng(20 + fast)
     for _ in range(50):
         f, rho, u, F = random_cell(oracle.D3Q27_CUDA, rng, True)
-        F = np.array([F[0], 0.0, 0.0]) if force == "x" else np.zeros(3)
+        if force == "x":
+            F = np.array([F[0], 0.0, 0.0])
+        elif force == "none":
+            F = np.zeros(3)
         got = hc_cell(hc, 27, 0, fast, f, rho, u, F, 0.57, mode)
         ref = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, 0.57, mode)
         assert np.abs(got - ref).max() < 1e-15

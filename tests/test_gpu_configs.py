@@ -63,7 +63,9 @@ def tg2d_run(rr, storage, steps_list, lx=100, ly=100, lz=4):
 def test_taylor_green_c1_energy_decay(rr):
     """100x100x4 quasi-2D Taylor-Green, Ma = 0.01, Re = 100, 2000 steps:
     FP64 GPU vs the FP64 oracle (fixture), vs the analytic decay
-    exp(-4 k^2 nu t), and FP32 / FP16 storage vs FP64 within 1e-4 relative."""
+    exp(-4 k^2 nu t), and FP32 / FP16 storage vs FP64 within 1e-4 relative
+    (FP16 with round-to-nearest misses this by 50x: the decay per step is
+    below half an FP16 step of the stored values)."""
     steps = [500, 1000, 1500, 2000]
     ke64, nu = tg2d_run(rr, 0, steps)
     ref = np.loadtxt(os.path.join(GOLDEN, "tg2d_c1_ke_oracle.txt"))
@@ -76,8 +78,8 @@ def test_taylor_green_c1_energy_decay(rr):
     assert np.abs(ke64 / ke64[0] - analytic).max() < 2e-3
     ke32, _ = tg2d_run(rr, 1, steps)
     assert np.abs(ke32 / ke64 - 1).max() < 1e-4
-    ke16, _ = tg2d_run(rr, 2, steps)
-    assert np.abs(ke16 / ke64 - 1).max() < 1e-2
+    ke16, _ = tg2d_run(rr, 2, steps)   # FP16 storage, stochastic rounding
+    assert np.abs(ke16 / ke64 - 1).max() < 1e-4
 
 
 @pytest.mark.parametrize("three_d", [False, True])
