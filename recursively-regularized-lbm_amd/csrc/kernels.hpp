@@ -59,7 +59,7 @@ constexpr int kDiagWidth = 9;  // sum(rho-1), sum u(3), sum rho u(3), sum KE, no
 struct FusedArgs {
   const void* src;      // post-collision populations of the previous step, SoA [Q][N]
   void* dst;            // post-collision populations of this step
-  const void* u_in;     // LES: velocity field read by the WALE stencil, SoA [3][N] (compute type)
+  const void* u_in;     // LES: velocity field read by the WALE stencil, AoS [N][3] (compute type)
   void* u_out;          // LES lagged / macro pass: velocity written by this launch
   double* ref_f;        // finish pass: reference-layout FP64 outputs
   double* ref_rho;

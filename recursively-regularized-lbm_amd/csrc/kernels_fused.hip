@@ -184,20 +184,30 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
   });
 }
 
+// LES velocity of one cell: the three components are stored together (AoS,
+// [N][3]) so that each stencil point is one 12-byte (FP32) load.
 template <typename C>
-RRLBM_INLINE __device__ C ld_u(const void* u, uint32_t n, int comp, uint32_t cell) {
-  return reinterpret_cast<const C*>(u)[(size_t)comp * n + cell];
+struct alignas(sizeof(C)) Vel3 {
+  C x, y, z;
+};
+
+template <typename C>
+RRLBM_INLINE __device__ Vel3<C> ld_v(const void* u, uint32_t cell) {
+  return reinterpret_cast<const Vel3<C>*>(u)[cell];
 }
 
-// One-sided wall derivative (LES.cu:19-33): d/dn at the first cell with the
-// half-way wall velocity taken as zero.
 template <typename C>
-RRLBM_INLINE __device__ C ddir(const void* u, uint32_t n, int comp, uint32_t c0, uint32_t cm, uint32_t cp, bool lo,
-                               bool hi) {
-  const C v0 = ld_u<C>(u, n, comp, c0);
-  if (lo) return (C(3) * v0 + ld_u<C>(u, n, comp, cp)) * C(1.0 / 3.0);
-  if (hi) return (C(-3) * v0 - ld_u<C>(u, n, comp, cm)) * C(1.0 / 3.0);
-  return C(0.5) * (ld_u<C>(u, n, comp, cp) - ld_u<C>(u, n, comp, cm));
+RRLBM_INLINE __device__ void st_v(void* u, uint32_t cell, C x, C y, C z) {
+  reinterpret_cast<Vel3<C>*>(u)[cell] = Vel3<C>{x, y, z};
+}
+
+// Derivative along one axis (LES.cu:19-33): central difference, or at a wall
+// cell the one-sided form with the half-way wall velocity taken as zero.
+template <typename C>
+RRLBM_INLINE __device__ C ddir(C v0, C vm, C vp, bool lo, bool hi) {
+  if (lo) return (C(3) * v0 + vp) * C(1.0 / 3.0);
+  if (hi) return (C(-3) * v0 - vm) * C(1.0 / 3.0);
+  return C(0.5) * (vp - vm);
 }
 
 // One workgroup's cells of one step: workgroup b of the domain described by a.
@@ -246,12 +256,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
 
   if constexpr (MODE == kMacro) {
-    if (active) {
-      C* u = reinterpret_cast<C*>(a.u_out);
-      u[cell] = q.ux;
-      u[(size_t)a.n + cell] = q.uy;
-      u[2 * (size_t)a.n + cell] = q.uz;
-    }
+    if (active) st_v<C>(a.u_out, cell, q.ux, q.uy, q.uz);
     return;
   } else if constexpr (MODE == kFinish) {
     if (active) {
@@ -307,23 +312,22 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       const bool ylo = wy && y == 0, yhi = wy && y == a.ly - 1;
       const bool zlo = wz && z == 0, zhi = wz && z == a.lz - 1;
       const void* u = a.u_in;
-      const C dudx = ddir<C>(u, a.n, 0, cell, cxm, cxp, xlo, xhi);
-      const C dvdx = ddir<C>(u, a.n, 1, cell, cxm, cxp, xlo, xhi);
-      const C dwdx = ddir<C>(u, a.n, 2, cell, cxm, cxp, xlo, xhi);
-      const C dudy = ddir<C>(u, a.n, 0, cell, cym, cyp, ylo, yhi);
-      const C dvdy = ddir<C>(u, a.n, 1, cell, cym, cyp, ylo, yhi);
-      const C dwdy = ddir<C>(u, a.n, 2, cell, cym, cyp, ylo, yhi);
-      const C dudz = ddir<C>(u, a.n, 0, cell, czm, czp, zlo, zhi);
-      const C dvdz = ddir<C>(u, a.n, 1, cell, czm, czp, zlo, zhi);
-      const C dwdz = ddir<C>(u, a.n, 2, cell, czm, czp, zlo, zhi);
+      const Vel3<C> v0 = ld_v<C>(u, cell);
+      const Vel3<C> vxm = ld_v<C>(u, cxm), vxp = ld_v<C>(u, cxp);
+      const Vel3<C> vym = ld_v<C>(u, cym), vyp = ld_v<C>(u, cyp);
+      const Vel3<C> vzm = ld_v<C>(u, czm), vzp = ld_v<C>(u, czp);
+      const C dudx = ddir<C>(v0.x, vxm.x, vxp.x, xlo, xhi);
+      const C dvdx = ddir<C>(v0.y, vxm.y, vxp.y, xlo, xhi);
+      const C dwdx = ddir<C>(v0.z, vxm.z, vxp.z, xlo, xhi);
+      const C dudy = ddir<C>(v0.x, vym.x, vyp.x, ylo, yhi);
+      const C dvdy = ddir<C>(v0.y, vym.y, vyp.y, ylo, yhi);
+      const C dwdy = ddir<C>(v0.z, vym.z, vyp.z, ylo, yhi);
+      const C dudz = ddir<C>(v0.x, vzm.x, vzp.x, zlo, zhi);
+      const C dvdz = ddir<C>(v0.y, vzm.y, vzp.y, zlo, zhi);
+      const C dwdz = ddir<C>(v0.z, vzm.z, vzp.z, zlo, zhi);
       const C tau_eff = C(a.tau) + wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
       omega = C(1) - recip(tau_eff);
-      if (a.write_u && active) {
-        C* uo = reinterpret_cast<C*>(a.u_out);
-        uo[cell] = q.ux;
-        uo[(size_t)a.n + cell] = q.uy;
-        uo[2 * (size_t)a.n + cell] = q.uz;
-      }
+      if (a.write_u && active) st_v<C>(a.u_out, cell, q.ux, q.uy, q.uz);
     }
     if constexpr (!LES) omega = C(a.omega);
     C out[Q];

@@ -398,9 +398,10 @@ template <typename C>
 __global__ __launch_bounds__(kBlock) void k_copy_u(RefFields r, Grid g, C* u) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
-  u[cell] = C(r.ux[cell]);
-  u[(size_t)g.n + cell] = C(r.uy[cell]);
-  u[2 * (size_t)g.n + cell] = C(r.uz[cell]);
+  // AoS [N][3], the fused path's LES velocity layout
+  u[3 * (size_t)cell] = C(r.ux[cell]);
+  u[3 * (size_t)cell + 1] = C(r.uy[cell]);
+  u[3 * (size_t)cell + 2] = C(r.uz[cell]);
 }
 
 // Per-plane statistics of output_profile (output.cu:182-215), one workgroup

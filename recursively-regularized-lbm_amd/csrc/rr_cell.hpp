@@ -480,7 +480,7 @@ RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz
   const C rs = sqrt(sijsq), rd = sqrt(sd);
   const C den = sijsq * sijsq * rs + sd * sqrt(rd);
   const C num = C(3) * cw * cw * sd * rd;
-  return den > C(0) ? num / den : C(0);
+  return den > C(0) ? num * recip(den) : C(0);
 }
 
 }  // namespace rrlbm
