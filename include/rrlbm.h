@@ -98,7 +98,17 @@ typedef struct rrlbm_params {
     int32_t cells_per_thread;   /* fused D3Q27 step without LES: 1 or 2 cells per thread
                                    (64 cells apart); 0 = the tuned / default choice */
     int32_t f16_rounding;       /* rrlbm_f16_rounding of the FP16 store (default stochastic) */
-    int32_t reserved[5];
+    int32_t pop_group_rows;     /* layout of the fused populations: groups of this many whole
+                                   rows (x lines) store their NPOP populations together;
+                                   -1 = one group (structure of arrays), 0 = library default
+                                   (groups of about 6144 cells when LX is a multiple of 64,
+                                   else one group); rrlbm_get_params reports the value used */
+    int32_t pop_pad_bytes;      /* bytes of padding after each population segment (multiple of 16,
+                                   at most 1 MiB); -1 = library default */
+    int32_t persistent_wg_per_cu; /* fused launches: 0 = one workgroup per block of cells;
+                                   k > 0 = a persistent grid of k workgroups per compute unit
+                                   striding over the blocks */
+    int32_t reserved[2];
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;
@@ -203,6 +213,28 @@ int rrlbm_fused_unpack(rrlbm_ctx* ctx);
 /* FP64 diagnostics of the fused state (end-of-step rho, u; no state change). */
 int rrlbm_fused_diag(rrlbm_ctx* ctx, rrlbm_diag* out);
 
+/* ------------------------------------------------------------ autotuning */
+/* Launch shape and population layout chosen for a context. */
+typedef struct rrlbm_tuning {
+    int32_t pop_group_rows;       /* as in rrlbm_params (value used) */
+    int32_t pop_pad_bytes;
+    int32_t cells_per_thread;
+    int32_t xcd_remap;
+    int32_t persistent_wg_per_cu;
+    int32_t candidates;           /* configurations timed; 0 = taken from this process's cache */
+    double ms_per_step;           /* fused step time of the choice in the trial */
+    double ms_per_step_default;   /* the library default's time in the same trial */
+} rrlbm_tuning;
+/* Times ctx's fused step over candidate population layouts (rows per group,
+ * padding) and launch shapes (1 or 2 cells per thread, linear or XCD-remapped
+ * block order, one workgroup per block or a persistent grid of 4 or 8 per
+ * compute unit), keeps the fastest for ctx and caches the choice per
+ * configuration (grid, lattice, storage, walls, LES, force components) for
+ * later calls in this process.  It steps ctx's own fused buffers, so it runs
+ * before rrlbm_fused_pack (RRLBM_E_STATE otherwise).  No reference counterpart:
+ * the reference fixes its 8x8x4 block at compile time (parameters.h:22-24, main.cu:26-27). */
+int rrlbm_autotune(rrlbm_ctx* ctx, int trial_steps, rrlbm_tuning* out);
+
 /* ------------------------------------------------------------ diagnostics */
 /* diag_flow sums (output.cu:3-66) over the reference fields. */
 int rrlbm_diag_flow(rrlbm_ctx* ctx, rrlbm_diag* out);
@@ -226,8 +258,9 @@ int rrlbm_release_reference(rrlbm_ctx* ctx);
 /* ------------------------------------------------------- batched ensembles */
 /* Many periodic D3Q27 boxes of uneven size, relaxation time and velocity
  * stepped together (no reference counterpart: the reference runs one domain
- * per process).  One launch per step covers every instance: a workgroup reads
- * its instance from a block -> instance table. */
+ * per process).  One launch per step covers every instance: the launch walks a
+ * work list of tiles (instance, block of 256 cells), one workgroup per tile or
+ * a persistent grid (rrlbm_batch_set_grid / rrlbm_batch_autotune). */
 typedef struct rrlbm_batch rrlbm_batch;
 typedef struct rrlbm_instance {
     int32_t lx, ly, lz;
@@ -258,6 +291,13 @@ int rrlbm_batch_graph_launch(rrlbm_batch* b, int times);
 int rrlbm_batch_time(rrlbm_batch* b, int mode, int nsteps, double* ms);
 /* FP64 diagnostics of one instance's current state. */
 int rrlbm_batch_diag(rrlbm_batch* b, int instance, rrlbm_diag* out);
+/* Grid of the batched launch: 0 = one workgroup per tile of the work list,
+ * k > 0 = a persistent grid of k workgroups per compute unit walking it. */
+int rrlbm_batch_set_grid(rrlbm_batch* b, int wg_per_cu);
+/* Times trial steps for grids of 0, 2, 4, 8 and 16 workgroups per compute unit
+ * and keeps the fastest; runs before rrlbm_batch_init_taylor_green_3d (it
+ * steps the instances' fused buffers). */
+int rrlbm_batch_autotune(rrlbm_batch* b, int trial_steps, int* wg_per_cu, double* ms_per_step);
 /* Borrowed context of one instance (valid until rrlbm_batch_destroy), e.g. to
  * unpack and download its fields. */
 int rrlbm_batch_context(rrlbm_batch* b, int instance, rrlbm_ctx** ctx);

@@ -55,9 +55,48 @@ struct StoreT<kF16> {
 
 constexpr int kDiagWidth = 9;  // sum(rho-1), sum u(3), sum rho u(3), sum KE, nonfinite count
 
+// Layout of the fused path's population buffers.  The cells (x fastest) are
+// split into groups of `grows` whole rows (gcells = grows*lx cells); a group
+// holds its Q populations one after another, each a segment of `seg` bytes
+// (gcells*sizeof + pad), so
+//   byte offset of population i of cell c = i*seg + c*sizeof + (c / gcells)*gpad,
+//   gpad = Q*seg - gcells*sizeof (the extra bytes per group).
+// One group (gcells = N) is structure-of-arrays with `pad` bytes between the
+// population arrays.  Groups of a few rows keep the populations one workgroup
+// reads and writes close together, which the HBM serves faster than Q arrays
+// a whole field apart.
+struct PopLayout {
+  uint64_t seg;        // bytes from population i to i+1 within a group
+  uint64_t gpad;       // extra bytes per group
+  uint64_t bytes;      // size of one population buffer
+  uint32_t grows;      // rows per group
+  uint32_t gcells;     // cells per group
+  FastDiv div_gcells;  // cell -> group
+  FastDiv div_grows;   // row (z*ly + y) -> group
+};
+// Byte offset of cell c within a population (without the term i*seg).  The
+// buffer of a layout with several groups is below 4 GiB (checked on the host);
+// with one group (structure of arrays) c / gcells = 0.
+RRLBM_HD RRLBM_INLINE uint32_t pop_cell_off(const PopLayout& l, uint32_t c, uint32_t sz) {
+  return c * sz + fastdiv(c, l.div_gcells) * (uint32_t)l.gpad;
+}
+// rows = ly*lz; grows in [1, rows]; pad in bytes (multiple of 16)
+inline PopLayout make_layout(uint32_t lx, uint32_t rows, int q, uint32_t sz, uint32_t grows, uint32_t pad) {
+  PopLayout l;
+  l.grows = grows;
+  l.gcells = grows * lx;
+  l.seg = (uint64_t)l.gcells * sz + pad;
+  l.gpad = (uint64_t)q * l.seg - (uint64_t)l.gcells * sz;
+  const uint64_t ngroups = (rows + grows - 1) / grows;
+  l.bytes = ngroups * q * l.seg;
+  l.div_gcells = make_fastdiv(l.gcells);
+  l.div_grows = make_fastdiv(grows);
+  return l;
+}
+
 // Arguments of the fused kernel (passed by value in the kernarg segment).
 struct FusedArgs {
-  const void* src;      // post-collision populations of the previous step, SoA [Q][N]
+  const void* src;      // post-collision populations of the previous step (layout `lay`)
   void* dst;            // post-collision populations of this step
   const void* u_in;     // LES: velocity field read by the WALE stencil, AoS [N][3] (compute type)
   void* u_out;          // LES lagged / macro pass: velocity written by this launch
@@ -83,7 +122,10 @@ struct FusedArgs {
   uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)
   uint32_t nblocks;     // workgroups covering the domain
   uint32_t blocks_per_xcd;
-  uint32_t xcd_remap;   // nonzero: launch is padded to 8*blocks_per_xcd and remapped per XCD
+  uint32_t xcd_remap;   // nonzero: block sequence padded to 8*blocks_per_xcd and remapped per XCD
+  uint32_t tiles;       // length of the block sequence (nblocks, or 8*blocks_per_xcd with the remap);
+                        // the grid may be smaller (persistent workgroups stride over it)
+  PopLayout lay;        // population buffer layout
 };
 
 // Host launchers (kernels_fused.hip)
@@ -91,8 +133,9 @@ hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, cons
                         hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
-hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
-                          uint32_t total_blocks, uint32_t parity, uint32_t step, hipStream_t s);
+// work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list
+hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
+                          uint32_t parity, uint32_t step, hipStream_t s);
 // Dither offset of batch instance k (the FP16 stochastic rounding of instance
 // k then differs from that of the other instances at the same cell index).
 inline uint32_t instance_salt(uint32_t k) { return k * 0x68E31DA5u; }

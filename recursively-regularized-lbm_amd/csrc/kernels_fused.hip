@@ -23,14 +23,15 @@
 
 namespace rrlbm {
 
-// Population i of the cell at byte offset `off` within one population array
-// (64-bit base of array i + 32-bit byte offset: one saddr global load).
+// Population i at byte offset `off` from `base` (the population buffer, or a
+// row of it); populations are `seg` bytes apart.  The 64-bit scalar part and
+// the 32-bit per-lane offset give one saddr global load.
 template <int S>
-RRLBM_INLINE __device__ typename StoreT<S>::C load_pop_b(const void* base, uint32_t n, int i, uint32_t off,
+RRLBM_INLINE __device__ typename StoreT<S>::C load_pop_b(const void* base, uint64_t seg, int i, uint32_t off,
                                                          typename StoreT<S>::C unscale) {
   using T = typename StoreT<S>::T;
   using C = typename StoreT<S>::C;
-  const char* p = reinterpret_cast<const char*>(base) + (size_t)i * n * sizeof(T);
+  const char* p = reinterpret_cast<const char*>(base) + (uint64_t)i * seg;
   const T v = *reinterpret_cast<const T*>(p + off);
   if constexpr (S == kF16)
     return C(v) * unscale;
@@ -39,29 +40,29 @@ RRLBM_INLINE __device__ typename StoreT<S>::C load_pop_b(const void* base, uint3
 }
 
 template <int S>
-RRLBM_INLINE __device__ typename StoreT<S>::C load_pop(const void* base, uint32_t n, int i, uint32_t cell,
-                                                       typename StoreT<S>::C unscale) {
-  return load_pop_b<S>(base, n, i, cell * (uint32_t)sizeof(typename StoreT<S>::T), unscale);
+RRLBM_INLINE __device__ uint32_t cell_off(const PopLayout& l, uint32_t c) {
+  return pop_cell_off(l, c, (uint32_t)sizeof(typename StoreT<S>::T));
 }
 
 // Post-collision populations are written once and read by the next launch:
 // non-temporal stores keep them from displacing the gather's cache lines.
 template <class T>
-RRLBM_INLINE __device__ void store_raw(void* base, uint32_t n, int i, uint32_t cell, T v) {
-  char* p = reinterpret_cast<char*>(base) + (size_t)i * n * sizeof(T);
-  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p + cell * (uint32_t)sizeof(T)));
+RRLBM_INLINE __device__ void store_raw(void* base, uint64_t seg, int i, uint32_t off, T v) {
+  char* p = reinterpret_cast<char*>(base) + (uint64_t)i * seg;
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p + off));
 }
 
-// Stochastic rounding of an FP32 value to FP16: the 13 dither bits (top bits
-// of r) are added to the 13 low mantissa bits that FP16 drops and the sum is
-// truncated.  The result is the upper FP16 neighbour with probability
-// (dropped part) / (FP16 step), so the stored value equals v in expectation;
-// the truncated value is exact in FP16 (normal range), so the conversion adds
-// no second rounding.  A carry into the exponent gives the next binade's
-// first value, as it should.
+// Stochastic rounding of an FP32 value to FP16 with gfx950's v_cvt_sr_f16_f32:
+// the 13 dither bits (top bits of r) are added to the 13 low mantissa bits that
+// FP16 drops and the sum is truncated, i.e. (bits(v) + (r >> 19)) & ~0x1FFF
+// (one instruction; it matched that integer form on 2^20 random (v, r) pairs
+// on MI355X).  The result is the upper FP16 neighbour with probability
+// (dropped part) / (FP16 step), so the stored value equals v in expectation.
+// A carry into the exponent gives the next binade's first value, as it should.
+typedef _Float16 rr_half2 __attribute__((ext_vector_type(2)));
 RRLBM_INLINE __device__ _Float16 f16_stochastic(float v, uint32_t r) {
-  const uint32_t b = (__float_as_uint(v) + (r >> 19)) & 0xFFFFE000u;
-  return (_Float16)__uint_as_float(b);
+  const rr_half2 z = {_Float16(0), _Float16(0)};
+  return __builtin_amdgcn_cvt_sr_f16_f32(z, v, r, false)[0];
 }
 
 // 32-bit integer mixer ("lowbias32", from C. Wellons' hash-prospector search).
@@ -83,23 +84,25 @@ RRLBM_INLINE __device__ uint32_t dither_base(uint32_t cell, uint32_t step, uint3
   return mix32(mix32(cell + salt) + step * 0x9E3779B9u);
 }
 
-// Store the Q post-collision values of one cell (FP16: scaled values).
+// Store the Q post-collision values of one cell (FP16: scaled values) at
+// base + i*seg + off.
 template <int S, int Q>
-RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, uint32_t cell, const typename StoreT<S>::C (&out)[Q]) {
+RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t off, uint32_t cell,
+                                        const typename StoreT<S>::C (&out)[Q]) {
   using T = typename StoreT<S>::T;
   if constexpr (S == kF16) {
     if (a.f16_round == 0) {
       const uint32_t r0 = dither_base(cell, a.sr_step, a.sr_salt);
       static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        store_raw<T>(a.dst, a.n, i, cell, f16_stochastic(out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
+        store_raw<T>(base, a.lay.seg, i, off, f16_stochastic(out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
       });
       return;
     }
   }
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    store_raw<T>(a.dst, a.n, i, cell, T(out[i]));
+    store_raw<T>(base, a.lay.seg, i, off, T(out[i]));
   });
 }
 
@@ -138,19 +141,20 @@ RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
 // post-collision opposite population of the same cell (+ moving-lid term
 // 6 w_i c_i.u_lid, Cavity/streaming.f90:142-152).
 template <int S, class L>
-RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, int i, int face, uint32_t cell,
-                                                       typename StoreT<S>::C inv_scale) {
+RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, const void* base, int i, int face,
+                                                       uint32_t off, typename StoreT<S>::C inv_scale) {
   using C = typename StoreT<S>::C;
   const int o = L::opp(i);
-  C v = load_pop<S>(a.src, a.n, o, cell, C(L::w(i)) * inv_scale);
+  C v = load_pop_b<S>(base, a.lay.seg, o, off, C(L::w(i)) * inv_scale);
   if (face == a.lid_face)
     v += C(6.0 * L::w(i)) * (C(L::cx(i)) * C(a.lid_x) + C(L::cy(i)) * C(a.lid_y) + C(L::cz(i)) * C(a.lid_z));
   return v;
 }
 
+// off: this cell's byte offset from `base` in the source buffer (without i*seg)
 template <int S, class L, int FM>
 RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
-                                                         uint32_t x, uint32_t y, uint32_t z, uint32_t cell) {
+                                            uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off) {
   using C = typename StoreT<S>::C;
   const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
@@ -163,7 +167,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
     if (!on[face]) return;
     static_for<L::Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, i, face, cell, inv_scale);
+      if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
     });
   });
   if (!a.wall_rr) return;
@@ -176,7 +180,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
     if (done) {
       static_for<L::Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, i, face, cell, inv_scale);
+        if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
       });
     }
     regularise_face<L, C, face, FM>(h, Fx, Fy, Fz);
@@ -210,43 +214,114 @@ RRLBM_INLINE __device__ C ddir(C v0, C vm, C vp, bool lo, bool hi) {
   return C(0.5) * (vp - vm);
 }
 
+// Byte offset (without the population term) of the first cell of row
+// (yy, zz): the whole rows before it plus the pads of the groups before it.
+RRLBM_INLINE __device__ uint64_t row_offset(const FusedArgs& a, uint32_t row_b, uint32_t yy, uint32_t zz) {
+  const uint32_t r = zz * a.ly + yy;
+  return (uint64_t)fastdiv(r, a.lay.div_grows) * a.lay.gpad + (uint64_t)r * row_b;
+}
+
+// Pull gather of the Q upwind populations of the cell (x, y, z), per-lane
+// addressing (any row length): nine source-row offsets, below 4 GiB.
+template <int S, class L>
+RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_t y, uint32_t z,
+                                         typename StoreT<S>::C (&h)[L::Q]) {
+  using C = typename StoreT<S>::C;
+  constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
+  const uint32_t row_b = a.lx * sz;
+  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  const uint32_t xb[3] = {(x + 1 == a.lx ? 0u : x + 1) * sz, x * sz, (x == 0 ? a.lx - 1 : x - 1) * sz};
+  const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
+  const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
+  uint32_t ro[3][3];
+  static_for<3>([&](auto jy) {
+    static_for<3>([&](auto jz) {
+      constexpr int iy = decltype(jy)::value, iz = decltype(jz)::value;
+      ro[iy][iz] = (uint32_t)row_offset(a, row_b, ys[iy], zs[iz]);
+    });
+  });
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+    h[i] = load_pop_b<S>(a.src, a.lay.seg, i, ro[cy + 1][cz + 1] + xb[cx + 1], C(L::w(i)) * inv_scale);
+  });
+}
+
 // One workgroup's cells of one step: workgroup b of the domain described by a.
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM>
+//
+// ROWS: the row length lx is a multiple of 64, so every wavefront covers 64
+// cells of one row.  Its y, z and the byte offsets of its nine source rows are
+// then wavefront-uniform (scalar registers): each population's source address
+// is a scalar row base plus one of three per-lane x offsets, instead of a
+// per-lane three-term sum.  Otherwise (any lx) every lane computes its own.
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS>
 RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
+  using T = typename StoreT<S>::T;
   constexpr int Q = L::Q;
-  const uint32_t raw = b * kBlock + threadIdx.x;
-  const bool active = raw < a.n;
-  const uint32_t cell = active ? raw : a.n - 1;
   const uint32_t lxly = a.lx * a.ly;
-  const uint32_t z = fastdiv(cell, a.div_lxly);
-  const uint32_t rem = cell - z * lxly;
-  const uint32_t y = fastdiv(rem, a.div_lx);
-  const uint32_t x = rem - y * a.lx;
+  uint32_t cell, x, y, z;
+  bool active;
+  if constexpr (ROWS) {
+    // n is a multiple of 64: a wavefront is wholly inside or wholly outside
+    uint32_t w0 = __builtin_amdgcn_readfirstlane(b * kBlock + (threadIdx.x & ~63u));
+    active = w0 < a.n;
+    if (!active) w0 = a.n - 64;
+    z = fastdiv(w0, a.div_lxly);
+    const uint32_t rem = w0 - z * lxly;
+    y = fastdiv(rem, a.div_lx);
+    const uint32_t lane = threadIdx.x & 63u;
+    x = rem - y * a.lx + lane;
+    cell = w0 + lane;
+  } else {
+    const uint32_t raw = b * kBlock + threadIdx.x;
+    active = raw < a.n;
+    cell = active ? raw : a.n - 1;
+    z = fastdiv(cell, a.div_lxly);
+    const uint32_t rem = cell - z * lxly;
+    y = fastdiv(rem, a.div_lx);
+    x = rem - y * a.lx;
+  }
   // upwind source coordinates for c = -1, 0, +1 (periodic wrap)
   const uint32_t xs[3] = {x + 1 == a.lx ? 0u : x + 1, x, x == 0 ? a.lx - 1 : x - 1};
   const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
   const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
   const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
-  // byte offsets of the source rows and columns: each population's source is
-  // one three-operand add
-  constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
-  const uint32_t row_b = a.lx * sz, plane_b = lxly * sz;
+  constexpr uint32_t sz = sizeof(T);
+  const uint32_t row_b = a.lx * sz;
   const uint32_t xb[3] = {xs[0] * sz, xs[1] * sz, xs[2] * sz};
-  const uint32_t yb[3] = {ys[0] * row_b, ys[1] * row_b, ys[2] * row_b};
-  const uint32_t zb[3] = {zs[0] * plane_b, zs[1] * plane_b, zs[2] * plane_b};
+  const char* src = reinterpret_cast<const char*>(a.src);
+  char* dst = reinterpret_cast<char*>(a.dst);
 
   C h[Q];
-  static_for<Q>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-    h[i] = load_pop_b<S>(a.src, a.n, i, zb[cz + 1] + yb[cy + 1] + xb[cx + 1], C(L::w(i)) * inv_scale);
-  });
+  if constexpr (ROWS) {
+    // nine scalar source-row bases, then the lane's x offset
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+      h[i] = load_pop_b<S>(src + row_offset(a, row_b, ys[cy + 1], zs[cz + 1]), a.lay.seg, i, xb[cx + 1],
+                           C(L::w(i)) * inv_scale);
+    });
+  } else {
+    gather_lane<S, L>(a, x, y, z, h);
+  }
+  // this cell in the source and destination buffers: base (scalar with ROWS) + offset
+  const char* own_src = src;
+  char* own_dst = dst;
+  uint32_t own_off;
+  if constexpr (ROWS) {
+    const uint64_t r0 = row_offset(a, row_b, y, z);
+    own_src += r0;
+    own_dst += r0;
+    own_off = x * sz;
+  } else {
+    own_off = cell_off<S>(a.lay, cell);
+  }
 
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-    if (on_wall) wall_treatment<S, L, FM>(a, h, x, y, z, cell);
+    if (on_wall) wall_treatment<S, L, FM>(a, h, x, y, z, own_src, own_off);
   }
 
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
@@ -299,6 +374,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
       a.partials[(size_t)b * kDiagWidth + threadIdx.x] = t;
     }
+    __syncthreads();  // red is reused by the workgroup's next block (persistent grids)
     return;
   } else {
     C omega;
@@ -347,16 +423,26 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
         });
       }
     }
-    if (active) store_cell<S, Q>(a, cell, out);
+    if (active) store_cell<S, Q>(a, own_dst, own_off, cell, out);
   }
 }
 
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM>
+// Block t of the launch's tile sequence -> block of cells: linear, or (XCD
+// remap) consecutive workgroup IDs, which the dispatcher deals round-robin to
+// the 8 XCDs, spread so that each XCD sweeps one contiguous 1/8 of the domain.
+RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
+  return a.xcd_remap ? (t & 7u) * a.blocks_per_xcd + (t >> 3) : t;
+}
+
+// One workgroup per block of 256 cells (grid = a.tiles), or a persistent grid
+// of fewer workgroups striding over the blocks (same result: every block is
+// computed once per launch).
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS>
 __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
-  uint32_t b = blockIdx.x;
-  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
-  if (b >= a.nblocks) return;
-  fused_body<S, L, WALLS, LES, MODE, FM>(a, b);
+  for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+    const uint32_t b = block_of(a, t);
+    if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS>(a, b);
+  }
 }
 
 // Two cells per thread (STEP mode, D3Q27, no LES): lane l of wavefront w of
@@ -365,19 +451,13 @@ __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
 // each wavefront keeps twice as many population loads in flight -- the better
 // layout when the per-cell arithmetic is a large share of the time.
 template <int S, bool WALLS, int FM>
-__global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
+RRLBM_INLINE __device__ void fused_pair_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
   using L = D3Q27;
   constexpr int Q = 27;
-  uint32_t b = blockIdx.x;
-  if (a.xcd_remap) b = (b & 7u) * a.blocks_per_xcd + (b >> 3);
-  if (b >= a.nblocks) return;
   const uint32_t base = b * (2 * kBlock) + (threadIdx.x >> 6) * 128 + (threadIdx.x & 63);
   const uint32_t craw[2] = {base, base + 64};
   const uint32_t lxly = a.lx * a.ly;
-  constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
-  const uint32_t row_b = a.lx * sz, plane_b = lxly * sz;
-  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
   C h[2][Q];
   uint32_t cell[2], cx_[2], cy_[2], cz_[2];
   bool active[2];
@@ -390,24 +470,18 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
     const uint32_t y = fastdiv(rem, a.div_lx);
     const uint32_t x = rem - y * a.lx;
     cx_[k] = x; cy_[k] = y; cz_[k] = z;
-    const uint32_t xb[3] = {(x + 1 == a.lx ? 0u : x + 1) * sz, x * sz, (x == 0 ? a.lx - 1 : x - 1) * sz};
-    const uint32_t yb[3] = {(y + 1 == a.ly ? 0u : y + 1) * row_b, y * row_b, (y == 0 ? a.ly - 1 : y - 1) * row_b};
-    const uint32_t zb[3] = {(z + 1 == a.lz ? 0u : z + 1) * plane_b, z * plane_b, (z == 0 ? a.lz - 1 : z - 1) * plane_b};
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      h[k][i] = load_pop_b<S>(a.src, a.n, i, zb[L::cz(i) + 1] + yb[L::cy(i) + 1] + xb[L::cx(i) + 1],
-                              C(L::w(i)) * inv_scale);
-    });
+    gather_lane<S, L>(a, x, y, z, h[k]);
   });
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
   const C omega = C(a.omega);
   static_for<2>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
+    const uint32_t off = cell_off<S>(a.lay, cell[k]);
     if constexpr (WALLS) {
       const uint32_t x = cx_[k], y = cy_[k], z = cz_[k];
       const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) ||
                            ((a.walls & 2) && (y == 0 || y == a.ly - 1)) || ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-      if (on_wall) wall_treatment<S, L, FM>(a, h[k], x, y, z, cell[k]);
+      if (on_wall) wall_treatment<S, L, FM>(a, h[k], x, y, z, a.src, off);
     }
     const Moments<C> m = moments_q27<C>(h[k]);
     const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
@@ -417,85 +491,110 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
       rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
     else
       rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
-    if (active[k]) store_cell<S, Q>(a, cell[k], out);
+    if (active[k]) store_cell<S, Q>(a, a.dst, off, cell[k], out);
   });
 }
 
-// Batched step over many periodic D3Q27 instances (ensembles): workgroup g of
-// the launch belongs to instance blk2inst[g]; each instance's domain,
-// relaxation time and A/B buffers come from its FusedArgs in `table`
-// (parity 1 swaps src and dst, so one captured graph alternates buffers;
-// `step` drives the FP16 stochastic-rounding dither).
-template <int S>
-__global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
-                                                         const uint32_t* __restrict__ blk2inst,
-                                                         const uint32_t* __restrict__ first_block,
-                                                         uint32_t total_blocks, uint32_t parity, uint32_t step) {
-  const uint32_t g = blockIdx.x;
-  if (g >= total_blocks) return;
-  const uint32_t inst = blk2inst[g];
-  FusedArgs a = table[inst];
-  a.sr_step = step;
-  if (parity) {
-    const void* t = a.src;
-    a.src = a.dst;
-    a.dst = const_cast<void*>(t);
+template <int S, bool WALLS, int FM>
+__global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
+  for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+    const uint32_t b = block_of(a, t);
+    if (b < a.nblocks) fused_pair_body<S, WALLS, FM>(a, b);
   }
-  fused_body<S, D3Q27, false, false, kStep, kForceNone>(a, g - first_block[inst]);
 }
 
-hipError_t launch_batched(int storage, const FusedArgs* table, const uint32_t* blk2inst, const uint32_t* first_block,
-                          uint32_t total_blocks, uint32_t parity, uint32_t step, hipStream_t s) {
-  const dim3 grid(total_blocks), block(kBlock);
+// Batched step over many periodic D3Q27 instances (ensembles).  The launch
+// walks a work list of tiles, work[t] = (instance, block of 256 cells of that
+// instance): with one workgroup per tile, or a persistent grid whose
+// workgroup g takes tiles g, g + grid, ...  Each instance's domain, relaxation
+// time, layout and A/B buffers come from its FusedArgs in `table` (parity 1
+// swaps src and dst, so one captured graph alternates buffers; `step` drives
+// the FP16 stochastic-rounding dither).
+template <int S>
+__global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
+                                                         const uint2* __restrict__ work, uint32_t ntiles,
+                                                         uint32_t parity, uint32_t step) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint2 w = work[t];
+    FusedArgs a = table[w.x];
+    a.sr_step = step;
+    if (parity) {
+      const void* s = a.src;
+      a.src = a.dst;
+      a.dst = const_cast<void*>(s);
+    }
+    if (a.lx % 64u == 0)
+      fused_body<S, D3Q27, false, false, kStep, kForceNone, true>(a, w.y);
+    else
+      fused_body<S, D3Q27, false, false, kStep, kForceNone, false>(a, w.y);
+  }
+}
+
+hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
+                          uint32_t parity, uint32_t step, hipStream_t s) {
+  const dim3 g(grid), block(kBlock);
   switch (storage) {
-    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
-    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
-    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, grid, block, 0, s, table, blk2inst, first_block, total_blocks, parity, step); break;
+    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, g, block, 0, s, table, work, ntiles, parity, step); break;
+    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, g, block, 0, s, table, work, ntiles, parity, step); break;
+    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, g, block, 0, s, table, work, ntiles, parity, step); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// Deterministic final sum of the per-workgroup diagnostics (fixed order).
-__global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* partials, uint32_t nparts, double* out) {
-  __shared__ double red[kBlock];
-  for (int k = 0; k < kDiagWidth; ++k) {
-    double t = 0.0;
-    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) t += partials[(size_t)p * kDiagWidth + k];
-    red[threadIdx.x] = t;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = red[0];
+// Deterministic final sum of the per-workgroup diagnostics: workgroup k sums
+// quantity k.  Thread t adds partials t, t + 1024, ... in four interleaved
+// accumulators (independent loads in flight), combined in a fixed order, then
+// a fixed-shape tree over the workgroup: the result does not depend on timing.
+constexpr int kFinBlock = 1024;
+__global__ __launch_bounds__(kFinBlock) void diag_finalize_kernel(const double* partials, uint32_t nparts, double* out) {
+  __shared__ double red[kFinBlock];
+  const uint32_t k = blockIdx.x;
+  double t4[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t p = threadIdx.x;
+  for (; p + 3 * kFinBlock < nparts; p += 4 * kFinBlock) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t4[j] += partials[(size_t)(p + j * kFinBlock) * kDiagWidth + k];
+  }
+  for (int j = 0; p < nparts; p += kFinBlock, ++j) t4[j] += partials[(size_t)p * kDiagWidth + k];
+  red[threadIdx.x] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+  __syncthreads();
+  for (int s = kFinBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = red[0];
 }
 
 hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(kBlock), 0, s, partials, nparts, out);
+  hipLaunchKernelGGL(diag_finalize_kernel, dim3(kDiagWidth), dim3(kFinBlock), 0, s, partials, nparts, out);
+  return hipGetLastError();
+}
+
+template <int S, class L, bool W, bool LES, int FM, bool R>
+static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  switch (mode) {
+    case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kFinish: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kFinish, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kDiag: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kDiag, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
 template <int S, class L, bool W, bool LES, int FM>
-static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
-  switch (mode) {
-    case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kFinish: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kFinish, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kDiag: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kDiag, FM>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+static hipError_t launch_rows(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  return a.lx % 64u == 0 ? launch_mode<S, L, W, LES, FM, true>(mode, a, grid, s)
+                         : launch_mode<S, L, W, LES, FM, false>(mode, a, grid, s);
 }
 
 template <int S, class L, bool W, bool LES>
 static hipError_t launch_force(int fm, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   switch (fm) {
-    case kForceNone: return launch_mode<S, L, W, LES, kForceNone>(mode, a, grid, s);
-    case kForceX: return launch_mode<S, L, W, LES, kForceX>(mode, a, grid, s);
-    default: return launch_mode<S, L, W, LES, kForceAll>(mode, a, grid, s);
+    case kForceNone: return launch_rows<S, L, W, LES, kForceNone>(mode, a, grid, s);
+    case kForceX: return launch_rows<S, L, W, LES, kForceX>(mode, a, grid, s);
+    default: return launch_rows<S, L, W, LES, kForceAll>(mode, a, grid, s);
   }
 }
 
@@ -512,8 +611,8 @@ static hipError_t launch_storage(int q, bool walls, bool les, int fm, int mode, 
   if (q == 19) {
     if (les) return hipErrorInvalidValue;  // WALE is provided on D3Q27 only
     // the D3Q19 path is the per-direction formulation with a general force
-    return walls ? launch_mode<S, D3Q19, true, false, kForceAll>(mode, a, grid, s)
-                 : launch_mode<S, D3Q19, false, false, kForceAll>(mode, a, grid, s);
+    return walls ? launch_rows<S, D3Q19, true, false, kForceAll>(mode, a, grid, s)
+                 : launch_rows<S, D3Q19, false, false, kForceAll>(mode, a, grid, s);
   }
   return hipErrorInvalidValue;
 }

@@ -361,8 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(RefFields r, Grid g, Ini
 // step's collision_RR with the uniform force of the parameters) and stored in
 // the fused kernel's storage.
 template <int S, class L>
-__global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst, double tau, int use_les, double fx,
-                                                 double fy, double fz, double f16_scale) {
+__global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst, PopLayout lay, double tau, int use_les,
+                                                 double fx, double fy, double fz, double f16_scale) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
   constexpr int Q = L::Q;
@@ -384,12 +384,13 @@ __global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst,
     rr_post_generic<L, double>(h, q, fx, fy, fz, omega, 1.0, out);
   }
   using T = typename StoreT<S>::T;
-  T* d = reinterpret_cast<T*>(dst);
+  // the fused path's population layout
+  char* d = reinterpret_cast<char*>(dst) + pop_cell_off(lay, cell, (uint32_t)sizeof(T));
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     double v = out[i];
     if constexpr (S == kF16) v = v * (f16_scale / L::w(i));
-    d[(size_t)i * g.n + cell] = T(v);
+    *reinterpret_cast<T*>(d + (uint64_t)i * lay.seg) = T(v);
   });
 }
 
@@ -535,9 +536,9 @@ hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStr
 hipError_t ref_init_fields(const RefFields& r, const Grid& g, const InitSpec& spec, hipStream_t s) {
   RRLBM_LAUNCH(k_init_fields, g, s, r, g, spec);
 }
-hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, double tau, int use_les,
-                    double fx, double fy, double fz, double f16_scale, hipStream_t s) {
-#define RRLBM_PACK(S, LT) RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, tau, use_les, fx, fy, fz, f16_scale)
+hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, const PopLayout& lay, double tau,
+                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s) {
+#define RRLBM_PACK(S, LT) RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, lay, tau, use_les, fx, fy, fz, f16_scale)
   if (q == 27) {
     if (storage == kF64) RRLBM_PACK(kF64, D3Q27);
     if (storage == kF32) RRLBM_PACK(kF32, D3Q27);

@@ -10,6 +10,9 @@
 #include <string.h>
 #include <sys/stat.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -32,13 +35,16 @@ struct rrlbm_ctx {
   double* f_temp = nullptr;
   void* pop_a = nullptr;  // post-collision populations read by the next step
   void* pop_b = nullptr;
-  size_t pop_bytes = 0;
+  size_t pop_bytes = 0;   // bytes of the current layout
+  size_t pop_cap = 0;     // bytes allocated per buffer (>= pop_bytes)
+  PopLayout lay{};        // layout of pop_a / pop_b
   void* u_a = nullptr;  // LES velocity buffers (compute type)
   void* u_b = nullptr;
   double* partials = nullptr;
   double* diag_out = nullptr;
   double* profiles = nullptr;
   uint32_t nblocks = 0;
+  int cus = 256;  // compute units of the device
   bool packed = false;
   uint32_t sr_step = 0;  // fused steps taken since creation (stochastic-rounding dither)
   bool ref_ok = true;  // reference-layout fields allocated
@@ -116,6 +122,51 @@ double* field_ptr(rrlbm_ctx* c, rrlbm_field f, size_t* count) {
 
 bool has_walls(const rrlbm_params& p) { return p.wall_x || p.wall_y || p.wall_z; }
 
+void set_blocks(FusedArgs& a, uint32_t nblocks, bool remap) {
+  a.nblocks = nblocks;
+  a.xcd_remap = remap ? 1u : 0u;
+  a.blocks_per_xcd = (nblocks + 7) / 8;
+  a.tiles = remap ? 8u * a.blocks_per_xcd : nblocks;
+}
+
+// Population layout from pop_group_rows / pop_pad_bytes; writes the values used
+// back into p.  Default: groups of about 6144 cells (whole rows) when rows are
+// whole wavefronts (LX a multiple of 64), padded by 256 B in FP16; otherwise
+// one group (structure of arrays) with 2 KiB between the arrays.  A layout of
+// several groups must stay below 4 GiB per buffer (32-bit lane offsets).
+int resolve_layout(rrlbm_params& p, PopLayout* out) {
+  const uint32_t rows = (uint32_t)p.ly * (uint32_t)p.lz;
+  const uint32_t sz = (uint32_t)storage_size(p.storage);
+  const int q = p.npop;
+  if (p.pop_group_rows < -1) return fail(nullptr, RRLBM_E_INVALID, "pop_group_rows must be -1, 0 or positive");
+  if (p.pop_pad_bytes < -1 || p.pop_pad_bytes > (1 << 20) || (p.pop_pad_bytes > 0 && p.pop_pad_bytes % 16))
+    return fail(nullptr, RRLBM_E_INVALID, "pop_pad_bytes must be -1 or a multiple of 16 in [0, 1 MiB]");
+  uint32_t grows;
+  if (p.pop_group_rows == -1)
+    grows = rows;
+  else if (p.pop_group_rows == 0)
+    grows = p.lx % 64 == 0 ? std::max(1u, std::min(rows, (6144u + (uint32_t)p.lx / 2) / (uint32_t)p.lx)) : rows;
+  else
+    grows = std::min<uint32_t>((uint32_t)p.pop_group_rows, rows);
+  uint32_t pad;
+  if (p.pop_pad_bytes >= 0)
+    pad = (uint32_t)p.pop_pad_bytes;
+  else
+    pad = grows == rows ? 2048u : (p.storage == RRLBM_STORE_F16 ? 256u : 0u);
+  PopLayout l = make_layout((uint32_t)p.lx, rows, q, sz, grows, pad);
+  if (grows < rows && l.bytes >= (1ull << 32)) {
+    if (p.pop_group_rows > 0)
+      return fail(nullptr, RRLBM_E_INVALID, "a population buffer of %d-row groups would exceed 4 GiB",
+                  p.pop_group_rows);
+    grows = rows;  // default: fall back to one group
+    l = make_layout((uint32_t)p.lx, rows, q, sz, grows, pad);
+  }
+  p.pop_group_rows = grows == rows ? -1 : (int32_t)grows;
+  p.pop_pad_bytes = (int32_t)pad;
+  *out = l;
+  return RRLBM_OK;
+}
+
 FusedArgs fused_args(rrlbm_ctx* c) {
   FusedArgs a{};
   a.src = c->pop_a;
@@ -132,19 +183,24 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.f16_scale = c->p.f16_scale;
   a.f16_round = (uint32_t)c->p.f16_rounding;
   a.sr_step = c->sr_step;
-  a.nblocks = c->nblocks;
-  a.xcd_remap = c->p.xcd_remap ? 1u : 0u;
-  a.blocks_per_xcd = (c->nblocks + 7) / 8;
+  set_blocks(a, c->nblocks, c->p.xcd_remap != 0);
   a.ref_f = c->ref.f; a.ref_rho = c->ref.rho; a.ref_ux = c->ref.ux; a.ref_uy = c->ref.uy; a.ref_uz = c->ref.uz;
   a.partials = c->partials;
+  a.lay = c->lay;
   return a;
 }
 
-uint32_t fused_grid(const rrlbm_ctx* c) { return c->p.xcd_remap ? 8u * ((c->nblocks + 7) / 8) : c->nblocks; }
+// Workgroups of a launch over a.tiles blocks: one per block, or a persistent
+// grid of persistent_wg_per_cu workgroups per compute unit striding over them.
+uint32_t fused_grid(const rrlbm_ctx* c, const FusedArgs& a) {
+  const int k = c->p.persistent_wg_per_cu;
+  if (k <= 0) return a.tiles;
+  return std::max(1u, std::min<uint32_t>(a.tiles, (uint32_t)k * (uint32_t)c->cus));
+}
 
 int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
   const bool les = c->p.use_les && mode == kStep;
-  hipError_t e = launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c), c->stream);
+  hipError_t e = launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a), c->stream);
   if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
   return RRLBM_OK;
 }
@@ -159,10 +215,8 @@ bool use_pair(const rrlbm_ctx* c) {
 int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
   if (use_pair(c)) {
     FusedArgs p = a;
-    p.nblocks = (c->n + 2 * kBlock - 1) / (2 * kBlock);
-    p.blocks_per_xcd = (p.nblocks + 7) / 8;
-    const uint32_t grid = c->p.xcd_remap ? 8u * p.blocks_per_xcd : p.nblocks;
-    hipError_t e = launch_fused_pair(c->p.storage, has_walls(c->p), p, grid, c->stream);
+    set_blocks(p, (c->n + 2 * kBlock - 1) / (2 * kBlock), c->p.xcd_remap != 0);
+    hipError_t e = launch_fused_pair(c->p.storage, has_walls(c->p), p, fused_grid(c, p), c->stream);
     if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
     return RRLBM_OK;
   }
@@ -282,6 +336,8 @@ int rrlbm_params_default(rrlbm_params* p) {
   p->cw = 0.5;                                 // LES.cu:12
   p->f16_scale = 4096.0;
   p->xcd_remap = 0;
+  p->pop_group_rows = 0;
+  p->pop_pad_bytes = -1;
   return RRLBM_OK;
 }
 
@@ -330,11 +386,20 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     if (!walled) return fail(nullptr, RRLBM_E_INVALID, "lid_face %d lies on a periodic axis", p.lid_face);
   }
   if (p.f16_scale == 0.0) p.f16_scale = 4096.0;
+  if (p.cells_per_thread < 0 || p.cells_per_thread > 2)
+    return fail(nullptr, RRLBM_E_INVALID, "cells_per_thread must be 0, 1 or 2");
+  if (p.persistent_wg_per_cu < 0 || p.persistent_wg_per_cu > 64)
+    return fail(nullptr, RRLBM_E_INVALID, "persistent_wg_per_cu must be in [0, 64]");
   if (p.f16_rounding != RRLBM_F16_STOCHASTIC && p.f16_rounding != RRLBM_F16_NEAREST)
     return fail(nullptr, RRLBM_E_INVALID, "f16_rounding must be 0 (stochastic) or 1 (nearest)");
   if (!(p.f16_scale > 0.0)) return fail(nullptr, RRLBM_E_INVALID, "f16_scale must be positive");
   const uint64_t per_pop = n64 * storage_size(p.storage);
   if (per_pop >= (1ull << 32)) return fail(nullptr, RRLBM_E_INVALID, "one population array exceeds 4 GiB");
+  PopLayout lay;
+  {
+    int rc = resolve_layout(p, &lay);
+    if (rc) return rc;
+  }
 
   rrlbm_ctx* c = new rrlbm_ctx();
   c->p = p;
@@ -347,6 +412,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   c->grid.lid_face = p.lid_face;
   c->grid.lid_x = p.lid_u[0]; c->grid.lid_y = p.lid_u[1]; c->grid.lid_z = p.lid_u[2];
   c->nblocks = (c->n + kBlock - 1) / kBlock;
+  c->lay = lay;
 
   auto bail = [&](int rc) {
     g_create_error = c->err;
@@ -354,6 +420,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     return rc;
   };
   hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -368,7 +435,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
                      (void**)&c->ref.fx,  (void**)&c->ref.fy, (void**)&c->ref.fz, (void**)&c->ref.taut};
   for (void** s : fields)
     if ((rc = dalloc(c, s, n * sizeof(double)))) return bail(rc);
-  c->pop_bytes = qn * storage_size(p.storage);
+  c->pop_bytes = c->pop_cap = lay.bytes;
   if ((rc = dalloc(c, &c->pop_a, c->pop_bytes))) return bail(rc);
   if ((rc = dalloc(c, &c->pop_b, c->pop_bytes))) return bail(rc);
   if (p.use_les) {
@@ -597,7 +664,7 @@ int rrlbm_init_equilibrium(rrlbm_ctx* c) {
 int rrlbm_fused_pack(rrlbm_ctx* c) {
   NEED_REF(c);
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
-  CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->p.tau, c->p.use_les, c->p.force[0],
+  CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->lay, c->p.tau, c->p.use_les, c->p.force[0],
                  c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream));
   if (c->p.use_les) CK(c, ref_copy_u(c->p.storage, c->ref, c->grid, c->u_a, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
@@ -705,6 +772,191 @@ int rrlbm_read_restart(rrlbm_ctx* c, const char* dir, int istep) {
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ autotuner
+// Picks, per configuration, the population layout (rows per group, padding)
+// and the launch shape (cells per thread, XCD-remapped block order,
+// persistent grid) of the fused step by timing trial steps on the context's
+// own buffers.  Coordinate search: layouts first (default launch shape), then
+// the launch shape with the best layout; every candidate is timed in three
+// interleaved rounds and scored by its fastest round.  Choices are cached per
+// configuration for the rest of the process.
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, rrlbm_tuning> g_tune_cache;
+
+int force_mode_key(const rrlbm_params& p) {
+  return (p.force[0] == 0.0 && p.force[1] == 0.0 && p.force[2] == 0.0) ? 0 : (p.force[1] == 0.0 && p.force[2] == 0.0) ? 1 : 2;
+}
+
+std::string tune_key(const rrlbm_params& p) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d", p.lx, p.ly, p.lz, p.npop, p.storage,
+           p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged, force_mode_key(p));
+  return buf;
+}
+
+int realloc_pops(rrlbm_ctx* c, size_t bytes) {
+  CK(c, hipStreamSynchronize(c->stream));
+  dfree(c->pop_a);
+  dfree(c->pop_b);
+  c->pop_a = c->pop_b = nullptr;
+  c->bytes -= 2 * c->pop_cap;
+  c->pop_cap = 0;
+  int rc = dalloc(c, &c->pop_a, bytes);
+  if (!rc) rc = dalloc(c, &c->pop_b, bytes);
+  if (rc) return rc;
+  c->pop_cap = bytes;
+  return RRLBM_OK;
+}
+
+// Switch ctx to the layout (group_rows, pad), growing the buffers if needed.
+int apply_layout(rrlbm_ctx* c, int group_rows, int pad) {
+  rrlbm_params p = c->p;
+  p.pop_group_rows = group_rows;
+  p.pop_pad_bytes = pad;
+  PopLayout l;
+  if (resolve_layout(p, &l)) return fail(c, RRLBM_E_INVALID, "%s", g_create_error.c_str());
+  if (l.bytes > c->pop_cap) {
+    int rc = realloc_pops(c, l.bytes);
+    if (rc) return rc;
+  }
+  c->lay = l;
+  c->pop_bytes = l.bytes;
+  c->p.pop_group_rows = p.pop_group_rows;
+  c->p.pop_pad_bytes = p.pop_pad_bytes;
+  return RRLBM_OK;
+}
+
+struct Candidate {
+  int group_rows, pad, cpt, remap, pers;
+};
+
+int apply_candidate(rrlbm_ctx* c, const Candidate& k) {
+  int rc = apply_layout(c, k.group_rows, k.pad);
+  if (rc) return rc;
+  c->p.cells_per_thread = k.cpt;
+  c->p.xcd_remap = k.remap;
+  c->p.persistent_wg_per_cu = k.pers;
+  return RRLBM_OK;
+}
+
+// Milliseconds per fused step over `steps` steps after two warm-up steps.
+int trial(rrlbm_ctx* c, int steps, double* ms) {
+  int rc = fused_steps(c, 2);
+  if (rc) return rc;
+  CK(c, hipEventRecord(c->ev0, c->stream));
+  rc = fused_steps(c, steps);
+  if (rc) return rc;
+  CK(c, hipEventRecord(c->ev1, c->stream));
+  CK(c, hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  CK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = (double)t / steps;
+  return RRLBM_OK;
+}
+
+// Best of three interleaved rounds for each candidate; returns the index of the fastest.
+int time_candidates(rrlbm_ctx* c, const std::vector<Candidate>& cand, int steps, std::vector<double>* best_ms,
+                    int* best) {
+  best_ms->assign(cand.size(), 1e300);
+  for (int round = 0; round < 3; ++round) {
+    for (size_t k = 0; k < cand.size(); ++k) {
+      int rc = apply_candidate(c, cand[k]);
+      if (rc) return rc;
+      double ms = 0.0;
+      rc = trial(c, steps, &ms);
+      if (rc) return rc;
+      (*best_ms)[k] = std::min((*best_ms)[k], ms);
+    }
+  }
+  *best = 0;
+  for (size_t k = 1; k < cand.size(); ++k)
+    if ((*best_ms)[k] < (*best_ms)[*best]) *best = (int)k;
+  return RRLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
+  NEED(c);
+  if (!out) return fail(c, RRLBM_E_INVALID, "null output");
+  if (trial_steps < 1 || trial_steps > 10000) return fail(c, RRLBM_E_INVALID, "trial_steps must be in [1, 10000]");
+  if (c->packed)
+    return fail(c, RRLBM_E_STATE, "rrlbm_autotune runs before rrlbm_fused_pack (it overwrites the fused buffers)");
+  memset(out, 0, sizeof *out);
+  const std::string key = tune_key(c->p);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *out = it->second;
+      out->candidates = 0;
+      return apply_candidate(c, Candidate{out->pop_group_rows, out->pop_pad_bytes, out->cells_per_thread,
+                                          out->xcd_remap, out->persistent_wg_per_cu});
+    }
+  }
+  const uint32_t sr0 = c->sr_step;
+  const uint32_t rows = (uint32_t)c->p.ly * (uint32_t)c->p.lz;
+  const bool rows64 = c->p.lx % 64 == 0;
+  // phase 1: layouts with the default launch shape (the library default first)
+  std::vector<Candidate> lays = {{0, -1, 1, 0, 0}, {-1, 0, 1, 0, 0}, {-1, 2048, 1, 0, 0}};
+  const int ks_rows[] = {1, 4, 16, 64};
+  const int ks_lane[] = {16};
+  const int* ks = rows64 ? ks_rows : ks_lane;
+  const int nks = rows64 ? 4 : 1;
+  for (int j = 0; j < nks; ++j)
+    if ((uint32_t)ks[j] < rows)
+      for (int pad : {0, 256}) lays.push_back({ks[j], pad, 1, 0, 0});
+  std::vector<double> ms1;
+  int b1 = 0;
+  int rc = time_candidates(c, lays, trial_steps, &ms1, &b1);
+  if (rc) return rc;
+  // phase 2: launch shapes with the best layout (resolved values, so the key is exact)
+  Candidate base = lays[b1];
+  rc = apply_candidate(c, base);
+  if (rc) return rc;
+  base.group_rows = c->p.pop_group_rows;
+  base.pad = c->p.pop_pad_bytes;
+  std::vector<Candidate> shapes;
+  const bool pair_ok = c->q == 27 && !c->p.use_les;
+  for (int cpt : {1, 2})
+    for (int remap : {0, 1})
+      for (int pers : {0, 4, 8}) {
+        if (cpt == 2 && !pair_ok) continue;
+        shapes.push_back({base.group_rows, base.pad, cpt, remap, pers});
+      }
+  std::vector<double> ms2;
+  int b2 = 0;
+  rc = time_candidates(c, shapes, trial_steps, &ms2, &b2);
+  if (rc) return rc;
+  const Candidate best = shapes[b2];
+  rc = apply_candidate(c, best);
+  if (rc) return rc;
+  if (c->pop_cap > c->pop_bytes) {  // shrink to the chosen layout
+    rc = realloc_pops(c, c->pop_bytes);
+    if (rc) return rc;
+  }
+  c->sr_step = sr0;
+  out->pop_group_rows = best.group_rows;
+  out->pop_pad_bytes = best.pad;
+  out->cells_per_thread = best.cpt;
+  out->xcd_remap = best.remap;
+  out->persistent_wg_per_cu = best.pers;
+  out->candidates = (int32_t)(lays.size() + shapes.size());
+  out->ms_per_step = ms2[b2];
+  out->ms_per_step_default = ms1[0];
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = *out;
+  }
+  return RRLBM_OK;
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------ batched ensembles
 struct rrlbm_batch {
   int storage = RRLBM_STORE_F32;
@@ -714,9 +966,10 @@ struct rrlbm_batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   FusedArgs* d_table = nullptr;
-  uint32_t* d_blk2inst = nullptr;
-  uint32_t* d_first = nullptr;
-  uint32_t total_blocks = 0;
+  uint2* d_work = nullptr;      // work list: (instance, block) per tile, instance-major
+  uint32_t total_blocks = 0;    // tiles in the work list
+  int wg_per_cu = 0;            // 0: one workgroup per tile; k: persistent grid of k per CU
+  int cus = 256;
   uint32_t parity = 0;  // 1: every instance's current state sits in its table dst buffer
   uint32_t sr_step = 0;  // eager batched / sequential steps taken (FP16 rounding dither)
   bool ready = false;   // instances initialised and packed, table uploaded
@@ -779,8 +1032,13 @@ int batch_upload_table(rrlbm_batch* b) {
   return RRLBM_OK;
 }
 
+uint32_t batch_grid(const rrlbm_batch* b) {
+  if (b->wg_per_cu <= 0) return b->total_blocks;
+  return std::max(1u, std::min<uint32_t>(b->total_blocks, (uint32_t)b->wg_per_cu * (uint32_t)b->cus));
+}
+
 int batch_launch(rrlbm_batch* b) {
-  hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, b->parity,
+  hipError_t e = launch_batched(b->storage, b->d_table, b->d_work, b->total_blocks, batch_grid(b), b->parity,
                                 b->sr_step, b->stream);
   if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "batched launch failed: %s", hipGetErrorString(e));
   ++b->sr_step;
@@ -795,7 +1053,7 @@ int batch_sequential(rrlbm_batch* b, int nsteps) {
       FusedArgs a = fused_args(c);
       a.sr_step = b->sr_step;
       a.sr_salt = instance_salt((uint32_t)k);
-      hipError_t e = launch_fused(c->p.storage, 27, false, false, kStep, a, fused_grid(c), b->stream);
+      hipError_t e = launch_fused(c->p.storage, 27, false, false, kStep, a, fused_grid(c, a), b->stream);
       if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "instance launch failed: %s", hipGetErrorString(e));
     }
     ++b->sr_step;
@@ -830,7 +1088,7 @@ int rrlbm_batch_create(const rrlbm_instance* inst, int count, int storage, int k
     rrlbm_batch_destroy(b);
     return rc;
   };
-  std::vector<uint32_t> blk2inst, first;
+  std::vector<uint2> work;
   for (int k = 0; k < count; ++k) {
     rrlbm_params p;
     rrlbm_params_default(&p);
@@ -846,23 +1104,19 @@ int rrlbm_batch_create(const rrlbm_instance* inst, int count, int storage, int k
       return bail(rc);
     }
     b->inst.push_back(c);
-    first.push_back((uint32_t)blk2inst.size());
-    for (uint32_t g = 0; g < c->nblocks; ++g) blk2inst.push_back((uint32_t)k);
+    for (uint32_t g = 0; g < c->nblocks; ++g) work.push_back(make_uint2((uint32_t)k, g));
     b->cells += c->n;
   }
   for (int k = 0; k < count; ++k) b->u0.push_back(inst[k].u0);
-  b->total_blocks = (uint32_t)blk2inst.size();
+  b->total_blocks = (uint32_t)work.size();
+  b->cus = b->inst[0]->cus;
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e == hipSuccess) e = hipMalloc(&b->d_table, count * sizeof(FusedArgs));
-  if (e == hipSuccess) e = hipMalloc(&b->d_blk2inst, blk2inst.size() * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&b->d_first, first.size() * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->d_work, work.size() * sizeof(uint2));
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_blk2inst, blk2inst.data(), blk2inst.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                       b->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream);
+    e = hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return bail(bfail(b, RRLBM_E_HIP, "batch setup failed: %s", hipGetErrorString(e)));
   *out = b;
@@ -876,8 +1130,7 @@ int rrlbm_batch_destroy(rrlbm_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (rrlbm_ctx* c : b->inst) rrlbm_destroy(c);
   dfree(b->d_table);
-  dfree(b->d_blk2inst);
-  dfree(b->d_first);
+  dfree(b->d_work);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -951,7 +1204,7 @@ int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps) {
   BCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   uint32_t par = start;
   for (int s = 0; s < nsteps; ++s) {
-    hipError_t e = launch_batched(b->storage, b->d_table, b->d_blk2inst, b->d_first, b->total_blocks, par,
+    hipError_t e = launch_batched(b->storage, b->d_table, b->d_work, b->total_blocks, batch_grid(b), par,
                                   b->sr_step + (uint32_t)s, b->stream);
     if (e != hipSuccess) {
       hipGraph_t g = nullptr;
@@ -1000,6 +1253,52 @@ int rrlbm_batch_diag(rrlbm_batch* b, int instance, rrlbm_diag* out) {
   rrlbm_ctx* c = b->inst[instance];
   int rc = rrlbm_fused_diag(c, out);
   if (rc) return bfail(b, rc, "instance %d: %s", instance, c->err.c_str());
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_set_grid(rrlbm_batch* b, int wg_per_cu) {
+  if (!b) return bfail(nullptr, RRLBM_E_INVALID, "null batch");
+  if (wg_per_cu < 0 || wg_per_cu > 64) return bfail(b, RRLBM_E_INVALID, "wg_per_cu must be in [0, 64]");
+  b->wg_per_cu = wg_per_cu;
+  return RRLBM_OK;
+}
+
+int rrlbm_batch_autotune(rrlbm_batch* b, int trial_steps, int* wg_per_cu, double* ms_per_step) {
+  if (!b) return bfail(nullptr, RRLBM_E_INVALID, "null batch");
+  if (trial_steps < 1 || trial_steps > 10000) return bfail(b, RRLBM_E_INVALID, "trial_steps must be in [1, 10000]");
+  if (b->ready)
+    return bfail(b, RRLBM_E_STATE, "rrlbm_batch_autotune runs before initialisation (it steps the fused buffers)");
+  for (rrlbm_ctx* c : b->inst)
+    if (c->packed) return bfail(b, RRLBM_E_STATE, "an instance is already packed");
+  int rc = batch_upload_table(b);
+  if (rc) return rc;
+  const uint32_t sr0 = b->sr_step;
+  const int cand[] = {0, 2, 4, 8, 16};
+  double best_ms[5];
+  for (double& v : best_ms) v = 1e300;
+  for (int round = 0; round < 3; ++round) {
+    for (int k = 0; k < 5; ++k) {
+      b->wg_per_cu = cand[k];
+      for (int w = 0; w < 2; ++w)
+        if ((rc = batch_launch(b))) return rc;
+      BCK(b, hipEventRecord(b->ev0, b->stream));
+      for (int st = 0; st < trial_steps; ++st)
+        if ((rc = batch_launch(b))) return rc;
+      BCK(b, hipEventRecord(b->ev1, b->stream));
+      BCK(b, hipEventSynchronize(b->ev1));
+      float t = 0.0f;
+      BCK(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+      best_ms[k] = std::min(best_ms[k], (double)t / trial_steps);
+    }
+  }
+  int best = 0;
+  for (int k = 1; k < 5; ++k)
+    if (best_ms[k] < best_ms[best]) best = k;
+  b->wg_per_cu = cand[best];
+  b->sr_step = sr0;
+  if (b->parity) batch_swap_all(b);  // back to parity 0 (the buffers hold no state yet)
+  if (wg_per_cu) *wg_per_cu = cand[best];
+  if (ms_per_step) *ms_per_step = best_ms[best];
   return RRLBM_OK;
 }
 

@@ -30,7 +30,8 @@ class Params(ctypes.Structure):
         ("tau", ctypes.c_double), ("force", ctypes.c_double * 3), ("cw", ctypes.c_double),
         ("lid_u", ctypes.c_double * 3), ("f16_scale", ctypes.c_double),
         ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("f16_rounding", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("pop_group_rows", ctypes.c_int32), ("pop_pad_bytes", ctypes.c_int32),
+        ("persistent_wg_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -56,6 +57,20 @@ class Diag(ctypes.Structure):
             "momentum": list(self.momentum), "kinetic_energy": self.kinetic_energy,
             "nonfinite_cells": self.nonfinite_cells, "ncells": self.ncells,
         }
+
+
+class Tuning(ctypes.Structure):
+    """rrlbm_tuning: the launch shape and population layout rrlbm_autotune chose."""
+
+    _fields_ = [
+        ("pop_group_rows", ctypes.c_int32), ("pop_pad_bytes", ctypes.c_int32),
+        ("cells_per_thread", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
+        ("persistent_wg_per_cu", ctypes.c_int32), ("candidates", ctypes.c_int32),
+        ("ms_per_step", ctypes.c_double), ("ms_per_step_default", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _P = ctypes.c_void_p
@@ -103,6 +118,9 @@ SIGNATURES = {
     "rrlbm_write_f": [_P, ctypes.c_char_p, _I],
     "rrlbm_read_restart": [_P, ctypes.c_char_p, _I],
     "rrlbm_release_reference": [_P],
+    "rrlbm_autotune": [_P, _I, ctypes.POINTER(Tuning)],
+    "rrlbm_batch_set_grid": [_P, _I],
+    "rrlbm_batch_autotune": [_P, _I, ctypes.POINTER(ctypes.c_int), _D],
     "rrlbm_batch_create": [ctypes.POINTER(Instance), _I, _I, _I, ctypes.POINTER(_P)],
     "rrlbm_batch_destroy": [_P],
     "rrlbm_batch_cells": [_P, ctypes.POINTER(ctypes.c_int64)],

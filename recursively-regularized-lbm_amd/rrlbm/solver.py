@@ -12,7 +12,7 @@ import enum
 
 import numpy as np
 
-from ._lib import Diag, Instance, Params, RRLBMError, check, check_batch, load
+from ._lib import Diag, Instance, Params, RRLBMError, Tuning, check, check_batch, load
 
 
 class Storage(enum.IntEnum):
@@ -256,6 +256,14 @@ class Solver:
     def release_reference(self):
         self._call("rrlbm_release_reference")
 
+    def autotune(self, trial_steps=20):
+        """Pick the fastest population layout and launch shape for this
+        configuration (before fused_pack); returns the choice as a dict."""
+        t = Tuning()
+        self._call("rrlbm_autotune", int(trial_steps), ctypes.byref(t))
+        self._call("rrlbm_get_params", ctypes.byref(self.params))
+        return t.as_dict()
+
 
 class _Borrowed(Solver):
     """A Solver view of one batch instance (owned by the batch)."""
@@ -316,6 +324,17 @@ class Batch:
 
     def init_taylor_green_3d(self):
         self._call("rrlbm_batch_init_taylor_green_3d")
+
+    def set_grid(self, wg_per_cu):
+        """0: one workgroup per tile; k: persistent grid of k workgroups per CU."""
+        self._call("rrlbm_batch_set_grid", int(wg_per_cu))
+
+    def autotune(self, trial_steps=10):
+        """Time the batched launch's grids (before init); returns (wg_per_cu, ms per step)."""
+        k = ctypes.c_int(0)
+        ms = ctypes.c_double(0.0)
+        self._call("rrlbm_batch_autotune", int(trial_steps), ctypes.byref(k), ctypes.byref(ms))
+        return k.value, ms.value
 
     def step(self, nsteps):
         self._call("rrlbm_batch_step", int(nsteps))

@@ -10,8 +10,10 @@ from helpers import maxabs
 
 pytestmark = pytest.mark.gpu
 
+# lx = 64 and 128: rows of whole wavefronts (row-scalar addressing, rows grouped in the
+# default population layout); the others take per-lane addressing
 INSTANCES = [(8, 8, 8, 0.51, 0.05), (12, 10, 9, 0.6, 0.03), (16, 16, 16, 0.55, 0.08), (7, 5, 6, 0.9, 0.02),
-             (20, 12, 10, 0.52, 0.06)]
+             (20, 12, 10, 0.52, 0.06), (64, 6, 5, 0.58, 0.04), (128, 50, 3, 0.53, 0.03)]
 
 
 def fields_of(b, k, rr):
@@ -26,10 +28,15 @@ def test_batched_equals_sequential_and_graph(rr, storage):
     only, so the three launch forms stay bit-identical."""
     steps = 7
     out = {}
-    for mode in ("batched", "sequential", "graph"):
+    for mode in ("batched", "sequential", "graph", "persistent1", "persistent3", "autotuned"):
         with rr.Batch(INSTANCES, storage=storage, keep_reference=True) as b:
+            if mode.startswith("persistent"):
+                b.set_grid(int(mode[-1]))   # 1 or 3 workgroups per CU walking the work list
+            if mode == "autotuned":
+                k, ms = b.autotune(trial_steps=2)
+                assert k in (0, 2, 4, 8, 16) and ms > 0
             b.init_taylor_green_3d()
-            if mode == "batched":
+            if mode in ("batched", "persistent1", "persistent3", "autotuned"):
                 b.step(steps)
             elif mode == "sequential":
                 b.step_sequential(steps)
@@ -38,8 +45,8 @@ def test_batched_equals_sequential_and_graph(rr, storage):
                 b.graph_launch(1)
             out[mode] = [fields_of(b, k, rr) for k in range(len(INSTANCES))]
     for k in range(len(INSTANCES)):
-        assert np.array_equal(out["batched"][k][0], out["sequential"][k][0])
-        assert np.array_equal(out["batched"][k][0], out["graph"][k][0])
+        for mode in ("sequential", "graph", "persistent1", "persistent3", "autotuned"):
+            assert np.array_equal(out["batched"][k][0], out[mode][k][0]), (k, mode)
 
 
 def test_batched_fp64_matches_oracle(rr):

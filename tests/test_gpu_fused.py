@@ -37,12 +37,29 @@ CASES = {
     "periodic_q19": (oracle.D3Q19, (10, 9, 7), dict(wall_y=0, wall_rr=0), (0, 0, 0), -1),
     "cavity2d_bb_only_q27": (oracle.D3Q27_CUDA, (10, 8, 2), dict(wall_x=1, wall_y=1, wall_rr=0, lid_face=1,
                                                                   lid_u=(0, 0.05, 0)), (1, 1, 0), 1),
+    # rows of 64 cells: each wavefront is one row (scalar row addressing); the
+    # cell counts leave the last workgroup partly empty
+    "channel_q27_rows64": (oracle.D3Q27_CUDA, (64, 6, 5), dict(wall_y=1, wall_rr=1), (0, 1, 0), -1),
+    "periodic_q27_rows64": (oracle.D3Q27_CUDA, (64, 5, 3), dict(wall_y=0, wall_rr=0), (0, 0, 0), -1),
+    "cavity_q27_rows128": (oracle.D3Q27_CUDA, (128, 5, 3), dict(wall_x=1, wall_y=1, wall_z=1, wall_rr=1, lid_face=3,
+                                                                lid_u=(0.05, 0, 0)), (1, 1, 1), 3),
+    "cavity_q19_rows64": (oracle.D3Q19, (64, 7, 5), dict(wall_x=1, wall_y=1, wall_z=1, wall_rr=1, lid_face=3,
+                                                         lid_u=(0.05, 0, 0)), (1, 1, 1), 3),
+    "periodic_q19_rows64": (oracle.D3Q19, (64, 5, 3), dict(wall_y=0, wall_rr=0), (0, 0, 0), -1),
 }
 
 
+# population layouts: library default, structure of arrays without padding,
+# groups of 3 rows with 48 B of padding (a partial last group in most cases)
+LAYOUTS = {"default": {}, "soa": dict(pop_group_rows=-1, pop_pad_bytes=0),
+           "rows3": dict(pop_group_rows=3, pop_pad_bytes=48)}
+
+
+@pytest.mark.parametrize("layout", sorted(LAYOUTS))
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_fused_fp64_matches_reference_order(rr, name):
+def test_fused_fp64_matches_reference_order(rr, name, layout):
     kind, dims, params, walls, lid = CASES[name]
+    params = dict(params, **LAYOUTS[layout])
     force = (0.0, 0.0, 0.0) if lid >= 0 else (3e-5, 1e-6, -2e-6)
     g = random_state(kind, *dims, seed=11, force=force)
     tau = 0.53
@@ -59,11 +76,12 @@ def test_fused_fp64_matches_reference_order(rr, name):
     assert np.allclose(diag["momentum"], ref[5:8], rtol=1e-10, atol=1e-13)
 
 
-@pytest.mark.parametrize("lagged", [0, 1])
-def test_fused_les(rr, lagged):
+@pytest.mark.parametrize("lagged,lx", [(0, 16), (1, 16), (0, 64), (1, 64)])
+def test_fused_les(rr, lagged, lx):
     """WALE in the fused path: reference order (two launches) is exact; the
-    one-launch lagged-gradient variant differs by the one-step lag only."""
-    g = channel_state(oracle.D3Q27_CUDA, 16, 12, 8, u_tau=0.01, visc=0.002, seed=7, force=(2e-5, 0, 0))
+    one-launch lagged-gradient variant differs by the one-step lag only
+    (lx = 64: rows of one wavefront)."""
+    g = channel_state(oracle.D3Q27_CUDA, lx, 12, 8, u_tau=0.01, visc=0.002, seed=7, force=(2e-5, 0, 0))
     tau = 0.5 + 3 * 0.002
     steps = 10
     got_f, got_m, _ = run_fused(rr, g, steps, storage=0, tau=tau, force=(2e-5, 0, 0), use_les=1,
@@ -76,10 +94,11 @@ def test_fused_les(rr, lagged):
         assert err < 1e-5  # lag of one step in tau_t only
 
 
-def test_fused_fp32_fp16_vs_fp64(rr):
+@pytest.mark.parametrize("lx", [32, 64])
+def test_fused_fp32_fp16_vs_fp64(rr, lx):
     """FP32 and FP16-shifted storage (FP32 arithmetic) against the FP64 run of
     the same kernels on a channel state, 50 steps."""
-    g = channel_state(oracle.D3Q27_CUDA, 32, 24, 16, u_tau=0.006, visc=0.0016, seed=3, force=(1e-5, 0, 0))
+    g = channel_state(oracle.D3Q27_CUDA, lx, 24, 16, u_tau=0.006, visc=0.0016, seed=3, force=(1e-5, 0, 0))
     tau = 0.5 + 3 * 0.0016
     res = {}
     for st in (0, 1, 2):
@@ -118,3 +137,87 @@ def test_two_cells_per_thread_matches_reference_order(rr, walls):
         f1, _, _ = run_fused(rr, g, 9, storage=st, cells_per_thread=1, **kw)
         f2, _, _ = run_fused(rr, g, 9, storage=st, cells_per_thread=2, **kw)
         assert maxabs(f1, f2) < tol
+
+
+@pytest.mark.parametrize("lx", [20, 64])
+@pytest.mark.parametrize("storage", [1, 2])
+@pytest.mark.parametrize("les", [0, 1])
+def test_population_layouts_are_bit_identical(rr, lx, storage, les):
+    """The population layout only moves bytes: structure of arrays, grouped
+    rows with a partial last group and padded groups give bit-identical
+    states (FP32 and FP16 storage, walls, LES, per-lane and row-scalar
+    addressing)."""
+    g = channel_state(oracle.D3Q27_CUDA, lx, 12, 7, u_tau=0.01, visc=0.002, seed=5, force=(2e-5, 0, 0))
+    tau = 0.5 + 3 * 0.002
+    out = []
+    for lay in (dict(pop_group_rows=-1, pop_pad_bytes=0), dict(pop_group_rows=5, pop_pad_bytes=0),
+                dict(pop_group_rows=2, pop_pad_bytes=256), {}):
+        f, m, _ = run_fused(rr, g, 15, storage=storage, tau=tau, force=(2e-5, 0, 0), wall_y=1, wall_rr=1,
+                            use_les=les, **lay)
+        out.append((f, m[1]))
+    for f, u in out[1:]:
+        assert np.array_equal(f, out[0][0])
+        assert np.array_equal(u, out[0][1])
+
+
+def test_layout_parameters_are_reported_and_checked(rr):
+    with rr.Solver(lx=128, ly=64, lz=8, storage=1) as s:
+        p = s.params
+        assert p.pop_group_rows == 48 and p.pop_pad_bytes == 0   # 48 rows of 128 = 6144 cells
+    with rr.Solver(lx=100, ly=10, lz=4, storage=2) as s:
+        p = s.params
+        assert p.pop_group_rows == -1 and p.pop_pad_bytes == 2048
+    for bad in (dict(pop_group_rows=-2), dict(pop_pad_bytes=24), dict(pop_pad_bytes=2 << 20)):
+        with pytest.raises(rr.RRLBMError) as ei:
+            rr.Solver(lx=8, ly=8, lz=8, **bad)
+        assert ei.value.status == 1
+
+
+@pytest.mark.parametrize("shape", [dict(persistent_wg_per_cu=1), dict(persistent_wg_per_cu=3, xcd_remap=1),
+                                   dict(cells_per_thread=2, persistent_wg_per_cu=1), dict(xcd_remap=1)])
+def test_launch_shapes_are_bit_identical(rr, shape):
+    """Persistent grids (workgroups striding over the blocks, diagnostics
+    included), the XCD-remapped block order and two cells per thread compute
+    the same state as one workgroup per block."""
+    g = channel_state(oracle.D3Q27_CUDA, 64, 24, 9, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 0, 0))
+    tau = 0.5 + 3 * 0.002
+    kw = dict(storage=1, tau=tau, force=(2e-5, 0, 0), wall_y=1, wall_rr=1)
+    f0, m0, d0 = run_fused(rr, g, 12, **kw)
+    f1, m1, d1 = run_fused(rr, g, 12, **kw, **shape)
+    if shape.get("cells_per_thread") == 2:
+        # the two-cell kernel is compiled separately: its FMA contractions may
+        # round differently (FP32: a few ulp)
+        assert maxabs(f0, f1) < 1e-6 and maxabs(m0[1], m1[1]) < 1e-6
+        return
+    assert np.array_equal(f0, f1) and np.array_equal(m0[1], m1[1])
+    assert d0["kinetic_energy"] == d1["kinetic_energy"] and d0["mass_excess"] == d1["mass_excess"]
+
+
+def test_autotune_picks_a_shape_with_identical_results(rr):
+    """rrlbm_autotune times layouts and launch shapes on the context's own
+    buffers before packing, keeps the fastest (identical arithmetic, so an
+    identical state), caches the choice, and refuses to run after packing."""
+    g = channel_state(oracle.D3Q27_CUDA, 128, 16, 8, u_tau=0.01, visc=0.002, seed=4, force=(2e-5, 0, 0))
+    tau = 0.5 + 3 * 0.002
+    kw = dict(storage=2, tau=tau, force=(2e-5, 0, 0), wall_y=1, wall_rr=1)
+    ref_f, ref_m, _ = run_fused(rr, g, 10, **kw)
+    with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, **kw) as s:
+        t = s.autotune(trial_steps=2)
+        assert t["candidates"] > 10 and t["ms_per_step"] > 0
+        p = s.params
+        assert (p.pop_group_rows, p.pop_pad_bytes, p.cells_per_thread, p.xcd_remap, p.persistent_wg_per_cu) == (
+            t["pop_group_rows"], t["pop_pad_bytes"], t["cells_per_thread"], t["xcd_remap"], t["persistent_wg_per_cu"])
+        upload_state(s, g)
+        s.fused_pack()
+        s.fused_step(9)
+        s.fused_unpack()
+        if t["cells_per_thread"] == 1:
+            assert np.array_equal(s.download(rr.Field.F), ref_f)
+        else:  # separately compiled two-cell kernel: rounding may differ (FP16 storage)
+            assert maxabs(s.download(rr.Field.F), ref_f) < 1e-3
+        with pytest.raises(rr.RRLBMError) as ei:
+            s.autotune(trial_steps=2)
+        assert ei.value.status == 3
+    with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, **kw) as s:
+        t2 = s.autotune(trial_steps=2)
+        assert t2["candidates"] == 0 and t2["pop_group_rows"] == t["pop_group_rows"]
