@@ -42,11 +42,12 @@ STORAGE_BYTES = {0: 8, 1: 4, 2: 2}
 STORAGE_NAME = {0: "f64", 1: "f32", 2: "f16"}
 
 
-def run_variant(rrlbm, storage, steps, warmup):
+def run_variant(rrlbm, storage, steps, warmup, tune=True):
     s = rrlbm.Solver(lx=LX, ly=LY, lz=LZ, npop=27, storage=storage, tau=TAU, force=(FORCE, 0.0, 0.0),
                      wall_y=1, wall_rr=1, use_les=0)
     try:
         s.init_channel(U_TAU, VISC, SEED, 0.1, 0.01)
+        tuning = s.autotune(trial_steps=10) if tune else None
         s.fused_pack()
         s.fused_step(warmup)
         s.synchronize()
@@ -71,6 +72,7 @@ def run_variant(rrlbm, storage, steps, warmup):
         "frac": achieved / PEAK_GBS,
         "mass_excess": d["mass_excess"],
         "ubulk_over_utau": d["u_sum"][0] / n / U_TAU,
+        "tuning": tuning,
     }
 
 
@@ -336,6 +338,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="library default layout and launch shape (no autotuning)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
     ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "configs", "validate", "copy"])
     ap.add_argument("--cavity-steps", type=int, default=20000)
@@ -357,7 +360,7 @@ def main():
 
     rrlbm.load()
     variants = [int(v) for v in args.variants.split(",")]
-    res = {st: run_variant(rrlbm, st, args.steps, args.warmup) for st in variants}
+    res = {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune) for st in variants}
     head = res[variants[0]]
     st0 = variants[0]
     line = {
@@ -383,7 +386,11 @@ def main():
         "variants": {STORAGE_NAME[st]: {"mlups": round(r["mlups"], 1), "frac": round(r["frac"], 4),
                                          "achieved_gbs": round(r["achieved_gbs"], 1),
                                          "bytes_per_lu": r["bytes_per_lu"], "kernel_ms": round(r["kernel_ms"], 5),
-                                         "ubulk_over_utau": round(r["ubulk_over_utau"], 4)}
+                                         "ubulk_over_utau": round(r["ubulk_over_utau"], 4),
+                                         "tuning": None if r["tuning"] is None else {
+                                             k: r["tuning"][k] for k in ("pop_group_rows", "pop_pad_bytes",
+                                                                         "cells_per_thread", "xcd_remap",
+                                                                         "persistent_wg_per_cu")}}
                      for st, r in res.items()},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }

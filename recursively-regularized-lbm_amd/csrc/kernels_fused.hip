@@ -434,14 +434,22 @@ RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
   return a.xcd_remap ? (t & 7u) * a.blocks_per_xcd + (t >> 3) : t;
 }
 
-// One workgroup per block of 256 cells (grid = a.tiles), or a persistent grid
-// of fewer workgroups striding over the blocks (same result: every block is
-// computed once per launch).
+// One workgroup per block of 256 cells (grid = a.tiles).
 template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS>
 __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
+  const uint32_t b = block_of(a, blockIdx.x);
+  if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS>(a, b);
+}
+
+// Persistent grid (step mode): fewer workgroups than blocks, each striding
+// over the block sequence; every block is computed once per launch.  A
+// separate kernel, because the loop raises the register count of the body
+// (loop-invariant addresses stay live), which one-block workgroups avoid.
+template <int S, class L, bool WALLS, bool LES, int FM, bool ROWS>
+__global__ __launch_bounds__(kBlock) void fused_kernel_persistent(FusedArgs a) {
   for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
     const uint32_t b = block_of(a, t);
-    if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS>(a, b);
+    if (b < a.nblocks) fused_body<S, L, WALLS, LES, kStep, FM, ROWS>(a, b);
   }
 }
 
@@ -495,10 +503,15 @@ RRLBM_INLINE __device__ void fused_pair_body(const FusedArgs& a, uint32_t b) {
   });
 }
 
-template <int S, bool WALLS, int FM>
+template <int S, bool WALLS, int FM, bool PERSIST>
 __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
-  for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-    const uint32_t b = block_of(a, t);
+  if constexpr (PERSIST) {
+    for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
+      const uint32_t b = block_of(a, t);
+      if (b < a.nblocks) fused_pair_body<S, WALLS, FM>(a, b);
+    }
+  } else {
+    const uint32_t b = block_of(a, blockIdx.x);
     if (b < a.nblocks) fused_pair_body<S, WALLS, FM>(a, b);
   }
 }
@@ -511,35 +524,50 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
 // swaps src and dst, so one captured graph alternates buffers; `step` drives
 // the FP16 stochastic-rounding dither).
 template <int S>
+RRLBM_INLINE __device__ void batched_tile(const FusedArgs* __restrict__ table, uint2 w, uint32_t parity,
+                                         uint32_t step) {
+  FusedArgs a = table[w.x];
+  a.sr_step = step;
+  if (parity) {
+    const void* s = a.src;
+    a.src = a.dst;
+    a.dst = const_cast<void*>(s);
+  }
+  if (a.lx % 64u == 0)
+    fused_body<S, D3Q27, false, false, kStep, kForceNone, true>(a, w.y);
+  else
+    fused_body<S, D3Q27, false, false, kStep, kForceNone, false>(a, w.y);
+}
+
+template <int S, bool PERSIST>
 __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
                                                          const uint2* __restrict__ work, uint32_t ntiles,
                                                          uint32_t parity, uint32_t step) {
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint2 w = work[t];
-    FusedArgs a = table[w.x];
-    a.sr_step = step;
-    if (parity) {
-      const void* s = a.src;
-      a.src = a.dst;
-      a.dst = const_cast<void*>(s);
-    }
-    if (a.lx % 64u == 0)
-      fused_body<S, D3Q27, false, false, kStep, kForceNone, true>(a, w.y);
-    else
-      fused_body<S, D3Q27, false, false, kStep, kForceNone, false>(a, w.y);
+  if constexpr (PERSIST) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) batched_tile<S>(table, work[t], parity, step);
+  } else {
+    if (blockIdx.x < ntiles) batched_tile<S>(table, work[blockIdx.x], parity, step);
   }
+}
+
+template <int S>
+static hipError_t launch_batched_s(const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
+                                   uint32_t parity, uint32_t step, hipStream_t s) {
+  if (grid < ntiles)
+    hipLaunchKernelGGL((batched_kernel<S, true>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step);
+  else
+    hipLaunchKernelGGL((batched_kernel<S, false>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step);
+  return hipGetLastError();
 }
 
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
                           uint32_t parity, uint32_t step, hipStream_t s) {
-  const dim3 g(grid), block(kBlock);
   switch (storage) {
-    case kF64: hipLaunchKernelGGL(batched_kernel<kF64>, g, block, 0, s, table, work, ntiles, parity, step); break;
-    case kF32: hipLaunchKernelGGL(batched_kernel<kF32>, g, block, 0, s, table, work, ntiles, parity, step); break;
-    case kF16: hipLaunchKernelGGL(batched_kernel<kF16>, g, block, 0, s, table, work, ntiles, parity, step); break;
+    case kF64: return launch_batched_s<kF64>(table, work, ntiles, grid, parity, step, s);
+    case kF32: return launch_batched_s<kF32>(table, work, ntiles, grid, parity, step, s);
+    case kF16: return launch_batched_s<kF16>(table, work, ntiles, grid, parity, step, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 // Deterministic final sum of the per-workgroup diagnostics: workgroup k sums
@@ -573,6 +601,10 @@ hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double*
 
 template <int S, class L, bool W, bool LES, int FM, bool R>
 static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  if (mode == kStep && grid < a.tiles) {
+    hipLaunchKernelGGL((fused_kernel_persistent<S, L, W, LES, FM, R>), dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+  }
   switch (mode) {
     case kStep: hipLaunchKernelGGL((fused_kernel<S, L, W, LES, kStep, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
     case kMacro: hipLaunchKernelGGL((fused_kernel<S, L, W, false, kMacro, FM, R>), dim3(grid), dim3(kBlock), 0, s, a); break;
@@ -617,14 +649,22 @@ static hipError_t launch_storage(int q, bool walls, bool les, int fm, int mode, 
   return hipErrorInvalidValue;
 }
 
+template <int S, bool W, int FM>
+static hipError_t launch_pair_grid(const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  if (grid < a.tiles)
+    hipLaunchKernelGGL((fused_pair_kernel<S, W, FM, true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((fused_pair_kernel<S, W, FM, false>), dim3(grid), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int S, bool W>
 static hipError_t launch_pair_force(int fm, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   switch (fm) {
-    case kForceNone: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceNone>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kForceX: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceX>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL((fused_pair_kernel<S, W, kForceAll>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kForceNone: return launch_pair_grid<S, W, kForceNone>(a, grid, s);
+    case kForceX: return launch_pair_grid<S, W, kForceX>(a, grid, s);
+    default: return launch_pair_grid<S, W, kForceAll>(a, grid, s);
   }
-  return hipGetLastError();
 }
 
 static int force_mode_of(const FusedArgs& a) {

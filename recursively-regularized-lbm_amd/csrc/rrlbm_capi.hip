@@ -130,10 +130,10 @@ void set_blocks(FusedArgs& a, uint32_t nblocks, bool remap) {
 }
 
 // Population layout from pop_group_rows / pop_pad_bytes; writes the values used
-// back into p.  Default: groups of about 6144 cells (whole rows) when rows are
-// whole wavefronts (LX a multiple of 64), padded by 256 B in FP16; otherwise
-// one group (structure of arrays) with 2 KiB between the arrays.  A layout of
-// several groups must stay below 4 GiB per buffer (32-bit lane offsets).
+// back into p.  Default: groups of about 6144 cells (whole rows, no padding)
+// when rows are whole wavefronts (LX a multiple of 64); otherwise one group
+// (structure of arrays) with 2 KiB between the arrays.  A layout of several
+// groups must stay below 4 GiB per buffer (32-bit lane offsets).
 int resolve_layout(rrlbm_params& p, PopLayout* out) {
   const uint32_t rows = (uint32_t)p.ly * (uint32_t)p.lz;
   const uint32_t sz = (uint32_t)storage_size(p.storage);
@@ -152,7 +152,7 @@ int resolve_layout(rrlbm_params& p, PopLayout* out) {
   if (p.pop_pad_bytes >= 0)
     pad = (uint32_t)p.pop_pad_bytes;
   else
-    pad = grows == rows ? 2048u : (p.storage == RRLBM_STORE_F16 ? 256u : 0u);
+    pad = grows == rows ? 2048u : 0u;
   PopLayout l = make_layout((uint32_t)p.lx, rows, q, sz, grows, pad);
   if (grows < rows && l.bytes >= (1ull << 32)) {
     if (p.pop_group_rows > 0)
