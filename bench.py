@@ -86,8 +86,9 @@ def traffic_from_profiles(storage):
     return t.get(STORAGE_NAME[storage], {}).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(steps=2):
-    """Oracle (C FP64, OpenMP) reference-order steps on the full grid."""
+def cpu_baseline(steps=20):
+    """Oracle (C FP64, OpenMP) reference-order steps on the full grid (about
+    10 s of host work at ~17 MLUPS on 16 cores)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 

@@ -94,7 +94,11 @@ typedef struct rrlbm_params {
     double lid_u[3];            /* lid velocity */
     double f16_scale;           /* FP16 storage: (f_i - w_i) * f16_scale / w_i is stored;
                                    0 selects 4096 */
-    int32_t xcd_remap;          /* 1: workgroup order remapped per XCD; 0: linear (default, faster on MI355X) */
+    int32_t xcd_remap;          /* order of the blocks of cells over the workgroup IDs, which the
+                                   dispatcher deals round-robin to the 8 XCDs: 0 linear;
+                                   1 one contiguous slab per XCD; C = 2^k (2..64) runs of C
+                                   consecutive blocks per XCD; -1 = library default (32 on grids
+                                   of at least 2048 blocks of 256 cells, else 0) */
     int32_t cells_per_thread;   /* fused D3Q27 step without LES: 1 or 2 cells per thread
                                    (64 cells apart); 0 = the tuned / default choice */
     int32_t f16_rounding;       /* rrlbm_f16_rounding of the FP16 store (default stochastic) */
@@ -226,9 +230,10 @@ typedef struct rrlbm_tuning {
     double ms_per_step_default;   /* the library default's time in the same trial */
 } rrlbm_tuning;
 /* Times ctx's fused step over candidate population layouts (rows per group,
- * padding) and launch shapes (1 or 2 cells per thread, linear or XCD-remapped
- * block order, one workgroup per block or a persistent grid of 4 or 8 per
- * compute unit), keeps the fastest for ctx and caches the choice per
+ * padding) and launch shapes (1 or 2 cells per thread, block order over the
+ * XCDs: linear, slabs, runs of 4 or 16 blocks; one workgroup per block or a
+ * persistent grid of 4 or 8 per compute unit), keeps the fastest for ctx and
+ * caches the choice per
  * configuration (grid, lattice, storage, walls, LES, force components) for
  * later calls in this process.  It steps ctx's own fused buffers, so it runs
  * before rrlbm_fused_pack (RRLBM_E_STATE otherwise).  No reference counterpart:

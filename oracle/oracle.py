@@ -175,6 +175,15 @@ class Fields:
         # (the C loop exchanges the contents of f and ftmp, so self.f holds the
         # streamed populations and self.ftmp the post-collision ones)
 
+    def step_laminar(self, nsteps, tau, stale_fbar=True):
+        """The Laminar forced-channel solver's step (Laminar/collision.f90:8-175,
+        main.f90:78-92): walls on both x faces, D3Q27 only; stale_fbar keeps
+        the reference's wall-loop Fbar (left from the sweep's last node)."""
+        assert self.kind == D3Q27_CUDA
+        load().orc_step_laminar(*self.dims, _I(nsteps), _p(self.f), _p(self.ftmp), _p(self.fx), _p(self.fy),
+                                _p(self.fz), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz),
+                                ctypes.c_double(tau), _I(int(stale_fbar)))
+
     def diag(self):
         out = np.zeros(9)
         load().orc_diag(ctypes.c_long(self.n), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz), _p(out))
@@ -185,6 +194,28 @@ class Fields:
         load().orc_profiles(*self.dims, _p(self.ux), _p(self.uy), _p(self.uz), _p(self.taut) if with_taut else None,
                             _p(out))
         return out.reshape(self.ly, 10)
+
+
+def laminar_parameters(nx=60):
+    """Laminar/para.f90:32-38: Rstar = 50, ustar = 0.05, visc = ustar nx / Rstar,
+    tau = 3 visc + 1/2, force_in_y = 8 visc ustar / nx^2 (FP64 arithmetic)."""
+    rstar, ustar = 50.0, 0.05
+    visc = ustar * float(nx) / rstar
+    tau = 3.0 * visc + 0.5
+    force_y = 8.0 * visc * ustar / float(nx) / float(nx)
+    return {"ustar": ustar, "visc": visc, "tau": tau, "force_y": force_y}
+
+
+def laminar_start(nx=60, ny=20, nz=2):
+    """Laminar/main.f90:34-55: u = 0, uniform force (0, force_y, 0), populations
+    = rest equilibrium w_i (initial.f90:2-31), then macrovar (u = F/2)."""
+    prm = laminar_parameters(nx)
+    g = Fields(D3Q27_CUDA, nx, ny, nz)
+    g.set_force((0.0, prm["force_y"], 0.0))
+    _, w, _ = lattice(D3Q27_CUDA)
+    g.f[:] = w[:, None]
+    g.macrovar()
+    return g, prm
 
 
 def uniform(seed, stream, counter):

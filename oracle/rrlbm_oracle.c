@@ -31,6 +31,8 @@
  *                                force on random linear fields
  *   orc_tg_ref_*              -> committed Fortran output
  *                                Taylor_Green/RRBMa0.01L100.txt (20 000 steps)
+ *   orc_step_laminar          -> committed FP64 fields Laminar/output/uy*.dat
+ *                                (5 000 and 10 000 steps, 1e-14 relative)
  *   orc_forcing_gp, orc_init_arrays -> closed-form spot values of the cited
  *                                formulas ("parity unpinned" beyond that: no
  *                                committed reference output exists)
@@ -170,17 +172,25 @@ void orc_feq(int kind, double rho, double u, double v, double w, double* feq) {
  *   mode 0 (collision, collision.cu:52-118): out_i = feq + (1 - 1/tau_eff) fneq + Fbar feq
  *   mode 1 (RRBC,      BC.cu:41-122):        out_i = feq + fneq - Fbar feq
  * f holds the cell's 27 (or 19) populations; the projection uses f + Fbar feq. */
-static void rr_cell(const orc_lat* L, const double* f, double rho, double u, double v, double w,
-                    double fx, double fy, double fz, double tau_eff, int mode, double* out) {
-    double feq[27], Fbar[27];
+/* Fbar_i = (c_i.F - u.F)/(2 cs^2), collision.cu:33-39 */
+static void fbar_cell(const orc_lat* L, double u, double v, double w, double fx, double fy, double fz, double* Fbar) {
     double G3 = u * fx + v * fy + w * fz;
+    for (int ip = 0; ip < L->q; ++ip) {
+        double G1 = L->cx[ip] * fx + L->cy[ip] * fy + L->cz[ip] * fz;
+        Fbar[ip] = 0.5 * (G1 - G3) / (1.0 / 3.0);
+    }
+}
+
+/* As rr_cell with the forcing vector Fbar given (the Laminar solver's wall
+ * regularisation passes a Fbar computed at another cell). */
+static void rr_cell_fbar(const orc_lat* L, const double* f, double rho, double u, double v, double w,
+                         const double* Fbar, double tau_eff, int mode, double* out) {
+    double feq[27];
     double uu = u / ORC_CS, vv = v / ORC_CS, ww = w / ORC_CS;
     orc_a a;
     eq_coeffs(uu, vv, ww, &a);
     double ane110 = 0, ane101 = 0, ane011 = 0, ane200 = 0, ane020 = 0, ane002 = 0;
     for (int ip = 0; ip < L->q; ++ip) {
-        double G1 = L->cx[ip] * fx + L->cy[ip] * fy + L->cz[ip] * fz;
-        Fbar[ip] = 0.5 * (G1 - G3) / (1.0 / 3.0);
         double fir = L->H100[ip] * uu + L->H010[ip] * vv + L->H001[ip] * ww;
         feq[ip] = L->w[ip] * rho * (1.0 + fir + 0.5 * (second_sum(L, ip, &a) + third_sum(L, ip, &a)));
         double g = f[ip] + Fbar[ip] * feq[ip];
@@ -205,6 +215,13 @@ static void rr_cell(const orc_lat* L, const double* f, double rho, double u, dou
         else
             out[ip] = feq[ip] + fneq - Fbar[ip] * feq[ip];
     }
+}
+
+static void rr_cell(const orc_lat* L, const double* f, double rho, double u, double v, double w,
+                    double fx, double fy, double fz, double tau_eff, int mode, double* out) {
+    double Fbar[27];
+    fbar_cell(L, u, v, w, fx, fy, fz, Fbar);
+    rr_cell_fbar(L, f, rho, u, v, w, Fbar, tau_eff, mode, out);
 }
 
 /* One-cell collision on caller arrays (for closed-form tests). */
@@ -527,6 +544,58 @@ void orc_step_channel(int kind, int lx, int ly, int lz, int nsteps, double* f, d
             orc_rrbc(kind, lx, ly, lz, f, ftmp, fx, fy, fz, rho, ux, uy, uz, wall_x, wall_y, wall_z, lid_face, lid_u);
             orc_macrovar(kind, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 1, wall_x, wall_y, wall_z);
         }
+    }
+}
+
+/* The Laminar forced-channel solver, Laminar/main.f90:78-92 with
+ * Laminar/collision.f90:8-175 (D3Q27, walls on the two x faces, periodic y
+ * and z, uniform body force).  One step of its collision_RR: every node
+ * collides with the macrovar state (collision.f90:30-61) and streams by swaps
+ * with half-way bounce-back at the x walls (collision.f90:63-103) -- restated
+ * here as push streaming + bounce-back, the same permutation -- then the
+ * wall-incoming populations of the x = 1 and x = LX nodes (c_x > 0 and c_x < 0)
+ * are replaced by feq + fneq - Fbar feq from the streamed populations, with
+ * u = (j + F/2)/rho (collision.f90:107-172); then macrovar (main.f90:86).
+ * stale_fbar != 0 keeps the reference's Fbar in the wall loops: the vector
+ * left over from the last node of the collision sweep (x = LX, y = LY, z = LZ,
+ * built from that node's pre-collision velocity, collision.f90:48-55); 0 uses
+ * each wall node's own u (as the channel's RRBC and the GPU library do). */
+void orc_step_laminar(int lx, int ly, int lz, int nsteps, double* f, double* ftmp, const double* fx,
+                      const double* fy, const double* fz, double* rho, double* ux, double* uy, double* uz,
+                      double tau, int stale_fbar) {
+    orc_lat L;
+    lat_build(ORC_D3Q27_CUDA, &L);
+    const long N = (long)lx * ly * lz;
+    for (int s = 0; s < nsteps; ++s) {
+        double Fst[27];
+        const long last = N - 1;
+        fbar_cell(&L, ux[last], uy[last], uz[last], fx[last], fy[last], fz[last], Fst);
+        orc_collision_rr(ORC_D3Q27_CUDA, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, NULL, tau);
+        orc_streaming(ORC_D3Q27_CUDA, lx, ly, lz, f, ftmp);
+        orc_bb(ORC_D3Q27_CUDA, lx, ly, lz, f, ftmp, 1, 0, 0, -1, NULL);
+        for (long i = 0; i < 27 * N; ++i) { double t = f[i]; f[i] = ftmp[i]; ftmp[i] = t; }
+        for (int side = 0; side < 2; ++side) {
+            const int ix = side == 0 ? 0 : lx - 1;
+            for (int iz = 0; iz < lz; ++iz)
+                for (int iy = 0; iy < ly; ++iy) {
+                    const long idx = cidx(ix, iy, iz, lx, ly);
+                    double fc[27], out[27], Fb[27];
+                    double r = 0, jx = 0, jy = 0, jz = 0;
+                    for (int ip = 0; ip < 27; ++ip) {
+                        fc[ip] = f[ip * N + idx];
+                        r += fc[ip]; jx += L.cx[ip] * fc[ip]; jy += L.cy[ip] * fc[ip]; jz += L.cz[ip] * fc[ip];
+                    }
+                    const double u = (jx + fx[idx] / 2.) / r, v = (jy + fy[idx] / 2.) / r, w = (jz + fz[idx] / 2.) / r;
+                    if (stale_fbar)
+                        for (int ip = 0; ip < 27; ++ip) Fb[ip] = Fst[ip];
+                    else
+                        fbar_cell(&L, u, v, w, fx[idx], fy[idx], fz[idx], Fb);
+                    rr_cell_fbar(&L, fc, r, u, v, w, Fb, 1.0, 1, out);
+                    for (int ip = 0; ip < 27; ++ip)
+                        if ((side == 0 && L.cx[ip] > 0) || (side == 1 && L.cx[ip] < 0)) f[ip * N + idx] = out[ip];
+                }
+        }
+        orc_macrovar(ORC_D3Q27_CUDA, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 0, 0, 0, 0);
     }
 }
 

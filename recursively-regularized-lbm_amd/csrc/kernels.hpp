@@ -122,8 +122,8 @@ struct FusedArgs {
   uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)
   uint32_t nblocks;     // workgroups covering the domain
   uint32_t blocks_per_xcd;
-  uint32_t xcd_remap;   // nonzero: block sequence padded to 8*blocks_per_xcd and remapped per XCD
-  uint32_t tiles;       // length of the block sequence (nblocks, or 8*blocks_per_xcd with the remap);
+  uint32_t xcd_remap;   // 0 linear block order; 1 one contiguous slab per XCD; C = 2^k: runs of C blocks per XCD
+  uint32_t tiles;       // length of the block sequence (nblocks rounded up for the remapped orders);
                         // the grid may be smaller (persistent workgroups stride over it)
   PopLayout lay;        // population buffer layout
 };

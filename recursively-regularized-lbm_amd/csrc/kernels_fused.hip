@@ -427,11 +427,17 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   }
 }
 
-// Block t of the launch's tile sequence -> block of cells: linear, or (XCD
-// remap) consecutive workgroup IDs, which the dispatcher deals round-robin to
-// the 8 XCDs, spread so that each XCD sweeps one contiguous 1/8 of the domain.
+// Block t of the launch's tile sequence -> block of cells.  The dispatcher
+// deals consecutive workgroup IDs round-robin to the 8 XCDs (each with its own
+// L2).  xcd_remap 0: linear; 1: each XCD sweeps one contiguous 1/8 of the
+// domain; C = 2^k > 1: each XCD takes runs of C consecutive blocks, so the
+// cache lines two neighbouring blocks share (the +-1 x shift of the gather)
+// mostly stay in one L2 while all XCDs still work on nearby memory.
 RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
-  return a.xcd_remap ? (t & 7u) * a.blocks_per_xcd + (t >> 3) : t;
+  if (a.xcd_remap == 0) return t;
+  if (a.xcd_remap == 1) return (t & 7u) * a.blocks_per_xcd + (t >> 3);
+  const uint32_t j = t >> 3, c = a.xcd_remap;
+  return ((j / c) * 8u + (t & 7u)) * c + (j & (c - 1u));
 }
 
 // One workgroup per block of 256 cells (grid = a.tiles).

@@ -16,6 +16,13 @@ Fixtures (plain text, data only):
       i, v, p of the oracle's pure-FP64 run of the same Taylor-Green case
       (for the FP64 GPU comparison; the Fortran output carries REAL*4
       intermediates).
+  laminar_60x20x2_uy_profiles.txt
+      the reference's committed FP64 fields of the Laminar forced channel
+      (Laminar/output/u{x,y,z}{5000,10000,60000}.dat, Fortran unformatted,
+      60x20x2, x fastest): the flow is uniform in y and z, so the file keeps
+      the x profile of u_y at each step (all digits) and, in the header, the
+      largest deviation of any node from its profile and the largest |u_x|,
+      |u_z|, so a test can check the whole field.
 """
 import os
 import sys
@@ -67,6 +74,34 @@ def tg2d_c1_ke(dst):
                header="oracle FP64 kinetic energy, Taylor-Green 100x100x4, u0 = 0.01 cs, Re = 100\ncolumns: step KE")
 
 
+REF_LAMINAR = "/root/reference/Laminar/output"
+
+
+def read_fortran_record(path, count):
+    """One Fortran unformatted sequential record of `count` FP64 values."""
+    raw = open(path, "rb").read()
+    n0 = int(np.frombuffer(raw[:4], dtype="<i4")[0])
+    n1 = int(np.frombuffer(raw[4 + n0:8 + n0], dtype="<i4")[0])
+    assert n0 == n1 == 8 * count and len(raw) == n0 + 8, (path, n0, n1, len(raw))
+    return np.frombuffer(raw[4:4 + n0], dtype="<f8").copy()
+
+
+def laminar_profiles(dst, nx=60, ny=20, nz=2):
+    steps = (5000, 10000, 60000)
+    prof, notes = [], []
+    for it in steps:
+        f = {c: read_fortran_record(os.path.join(REF_LAMINAR, "u%s%d.dat" % (c, it)), nx * ny * nz).reshape(nz, ny, nx)
+             for c in "xyz"}
+        p = f["y"][0, 0, :]
+        dev = float(np.abs(f["y"] - p[None, None, :]).max())
+        notes.append("step %d: max |uy - profile| = %.3e, max |ux| = %.3e, max |uz| = %.3e"
+                     % (it, dev, float(np.abs(f["x"]).max()), float(np.abs(f["z"]).max())))
+        prof.append(p)
+    hdr = ("Laminar forced channel 60x20x2 (Laminar/), reference FP64 output u_y(x) at steps 5000, 10000, 60000\n"
+           + "\n".join(notes) + "\ncolumns: ix uy_5000 uy_10000 uy_60000")
+    np.savetxt(dst, np.column_stack([np.arange(nx)] + prof), fmt=["%d", "%.17e", "%.17e", "%.17e"], header=hdr)
+
+
 def main():
     fortran_midline(REF_TG27, os.path.join(HERE, "tg_d3q27_L100_it20000_fortran_midline.txt"),
                     "from Taylor_Green/RRBMa0.01L100.txt (reference output)")
@@ -78,6 +113,7 @@ def main():
                     "from Taylor_Green/D3Q19RR.f90 built from source (amdflang -O2, default REAL kind 4)")
     oracle_fp64_midline(os.path.join(HERE, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
     tg2d_c1_ke(os.path.join(HERE, "tg2d_c1_ke_oracle.txt"))
+    laminar_profiles(os.path.join(HERE, "laminar_60x20x2_uy_profiles.txt"))
 
 
 if __name__ == "__main__":

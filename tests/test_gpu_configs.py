@@ -156,3 +156,49 @@ def test_restart_files_roundtrip(rr, tmp_path):
         with pytest.raises(rr.RRLBMError) as ei:
             s2.read_restart(d, 999)
         assert ei.value.status == 5
+
+
+def laminar_solver(rr, storage, **kw):
+    g, prm = oracle.laminar_start()
+    s = rr.Solver(lx=60, ly=20, lz=2, npop=27, storage=storage, tau=prm["tau"], force=(0.0, prm["force_y"], 0.0),
+                  wall_x=1, wall_y=0, wall_z=0, wall_rr=1, **kw)
+    upload_state(s, g)
+    return s, g, prm
+
+
+def test_laminar_channel_against_reference_output(rr):
+    """The Laminar forced channel (Laminar/: 60x20x2, walls at the x faces,
+    uniform force in y) on the fused FP64 path: equal to the oracle's
+    reference-order restatement (own Fbar at the wall) to 1e-13 after 5 000
+    steps, and to the reference's committed FP64 fields after 5 000, 10 000
+    and 60 000 steps within the effect of the reference's left-over wall Fbar
+    (measured on the CPU: 9.4e-12, 3.0e-11, 2.7e-10 of u_y).  FP32 and FP16
+    storage stay within 0.5 % of the FP64 profile's maximum (the channel
+    tolerance; measured after 10 000 steps: FP32 4.8e-6 = 0.07 %, FP16
+    2.2e-5 = 0.3 % of the 6.9e-3 maximum) -- a flow that changes by about
+    one part in 1e4 per step accumulates the FP32 rounding of every step."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "laminar_60x20x2_uy_profiles.txt"))
+    got = {}
+    for st in (0, 1, 2):
+        s, g, prm = laminar_solver(rr, st)
+        with s:
+            s.fused_pack()
+            done, out = 1, []
+            for it in (5000, 10000, 60000):
+                s.fused_step(it - done)
+                done = it
+                s.fused_unpack()
+                out.append(s.macroscopic())
+            got[st] = out
+    g, prm = oracle.laminar_start()
+    g.step_laminar(5000, prm["tau"], stale_fbar=False)
+    assert maxabs(got[0][0][2], g.uy) < 1e-13
+    assert maxabs(got[0][0][1], g.ux) < 1e-13
+    for k, (col, tol) in enumerate(((1, 2e-11), (2, 6e-11), (3, 5e-10))):
+        uy = got[0][k][2].reshape(2, 20, 60)
+        print("laminar step", (5000, 10000, 60000)[k], "fp64 vs fixture", np.abs(uy - ref[None, None, :, col]).max(),
+              "fp32", maxabs(got[1][k][2], got[0][k][2]), "fp16", maxabs(got[2][k][2], got[0][k][2]))
+        assert np.abs(uy - ref[None, None, :, col]).max() < tol
+        umax = np.abs(got[0][k][2]).max()
+        for st in (1, 2):
+            assert maxabs(got[st][k][2], got[0][k][2]) < 5e-3 * umax, (st, k, maxabs(got[st][k][2], got[0][k][2]))
