@@ -211,6 +211,16 @@ int rrlbm_fused_step(rrlbm_ctx* ctx, int nsteps);
 /* As rrlbm_fused_step, timing the launches with HIP events on the context's
  * stream; *ms receives the elapsed milliseconds. */
 int rrlbm_fused_step_timed(rrlbm_ctx* ctx, int nsteps, double* ms);
+/* Capture nsteps fused steps into a HIP graph (replaces a previous capture);
+ * the reference launches every kernel from the host each step (main.cu:31-117).
+ * Replays must start from the buffer the capture started from: with an odd
+ * nsteps every second replay is refused (RRLBM_E_STATE; capture again).  With
+ * FP16 storage every replay repeats the captured steps' rounding dither. */
+int rrlbm_graph_capture(rrlbm_ctx* ctx, int nsteps);
+/* Replay the captured graph `times` times (times * nsteps steps). */
+int rrlbm_graph_launch(rrlbm_ctx* ctx, int times);
+/* As rrlbm_graph_launch, timed with HIP events on the context's stream (ms). */
+int rrlbm_graph_launch_timed(rrlbm_ctx* ctx, int times, double* ms);
 /* Finish the current step into the reference fields: streaming, BB, macrovar,
  * RRBC and macrovar_BN, writing f, rho, u (FP64). */
 int rrlbm_fused_unpack(rrlbm_ctx* ctx);

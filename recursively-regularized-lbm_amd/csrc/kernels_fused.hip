@@ -90,8 +90,13 @@ template <int S, int Q>
 RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t off, uint32_t cell,
                                         const typename StoreT<S>::C (&out)[Q]) {
   using T = typename StoreT<S>::T;
+  // Each branch takes a fresh 32-bit copy of the lane offset: the offset is
+  // also used by the wall path, and a 64-bit extension of it shared across
+  // blocks turns every store into a per-lane 64-bit address add instead of a
+  // scalar-base + 32-bit-offset store.
   if constexpr (S == kF16) {
     if (a.f16_round == 0) {
+      asm volatile("" : "+v"(off));
       const uint32_t r0 = dither_base(cell, a.sr_step, a.sr_salt);
       static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -100,6 +105,7 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
       return;
     }
   }
+  asm volatile("" : "+v"(off));
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     store_raw<T>(base, a.lay.seg, i, off, T(out[i]));

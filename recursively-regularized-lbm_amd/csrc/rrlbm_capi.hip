@@ -47,6 +47,10 @@ struct rrlbm_ctx {
   int cus = 256;  // compute units of the device
   bool packed = false;
   uint32_t sr_step = 0;  // fused steps taken since creation (stochastic-rounding dither)
+  hipGraph_t graph = nullptr;  // captured fused steps (rrlbm_graph_capture)
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  void* graph_pop_a = nullptr;  // buffer the captured sequence starts from
   bool ref_ok = true;  // reference-layout fields allocated
   size_t bytes = 0;
   std::string err;
@@ -473,6 +477,8 @@ int rrlbm_destroy(rrlbm_ctx* c) {
                   c->ref.fz, c->ref.taut, c->pop_a, c->pop_b, c->u_a, c->u_b, c->partials, c->diag_out,
                   c->profiles};
   for (void* p : ptrs) dfree(p);
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -697,6 +703,70 @@ int rrlbm_fused_step_timed(rrlbm_ctx* c, int nsteps, double* ms) {
   if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
   CK(c, hipEventRecord(c->ev0, c->stream));
   int rc = fused_steps(c, nsteps);
+  if (rc) return rc;
+  CK(c, hipEventRecord(c->ev1, c->stream));
+  CK(c, hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  CK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = (double)t;
+  return RRLBM_OK;
+}
+
+int rrlbm_graph_capture(rrlbm_ctx* c, int nsteps) {
+  NEED(c);
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before capturing fused steps");
+  if (nsteps < 1 || nsteps > 100000) return fail(c, RRLBM_E_INVALID, "nsteps must be in [1, 100000]");
+  CK(c, hipStreamSynchronize(c->stream));
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  // the launches record the buffer swaps and the dither counter on the host;
+  // nothing runs during capture, so restore them afterwards
+  void* const a0 = c->pop_a;
+  void* const b0 = c->pop_b;
+  void* const ua0 = c->u_a;
+  void* const ub0 = c->u_b;
+  const uint32_t sr0 = c->sr_step;
+  CK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = fused_steps(c, nsteps);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->pop_a = a0; c->pop_b = b0; c->u_a = ua0; c->u_b = ub0; c->sr_step = sr0;
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    return fail(c, RRLBM_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  }
+  c->graph = g;
+  CK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+  c->graph_steps = nsteps;
+  c->graph_pop_a = a0;
+  return RRLBM_OK;
+}
+
+int rrlbm_graph_launch(rrlbm_ctx* c, int times) {
+  NEED(c);
+  if (!c->exec) return fail(c, RRLBM_E_STATE, "no captured graph");
+  if (times < 0) return fail(c, RRLBM_E_INVALID, "times must be >= 0");
+  for (int t = 0; t < times; ++t) {
+    if (c->pop_a != c->graph_pop_a)
+      return fail(c, RRLBM_E_STATE, "the state sits in the other buffer than at capture (odd step count); capture again");
+    CK(c, hipGraphLaunch(c->exec, c->stream));
+    for (int s = 0; s < c->graph_steps; ++s) {
+      std::swap(c->pop_a, c->pop_b);
+      if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
+    }
+    c->sr_step += (uint32_t)c->graph_steps;
+  }
+  return RRLBM_OK;
+}
+
+int rrlbm_graph_launch_timed(rrlbm_ctx* c, int times, double* ms) {
+  NEED(c);
+  if (!ms) return fail(c, RRLBM_E_INVALID, "null output");
+  CK(c, hipEventRecord(c->ev0, c->stream));
+  int rc = rrlbm_graph_launch(c, times);
   if (rc) return rc;
   CK(c, hipEventRecord(c->ev1, c->stream));
   CK(c, hipEventSynchronize(c->ev1));

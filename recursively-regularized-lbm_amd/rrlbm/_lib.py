@@ -119,6 +119,9 @@ SIGNATURES = {
     "rrlbm_read_restart": [_P, ctypes.c_char_p, _I],
     "rrlbm_release_reference": [_P],
     "rrlbm_autotune": [_P, _I, ctypes.POINTER(Tuning)],
+    "rrlbm_graph_capture": [_P, _I],
+    "rrlbm_graph_launch": [_P, _I],
+    "rrlbm_graph_launch_timed": [_P, _I, _D],
     "rrlbm_batch_set_grid": [_P, _I],
     "rrlbm_batch_autotune": [_P, _I, ctypes.POINTER(ctypes.c_int), _D],
     "rrlbm_batch_create": [ctypes.POINTER(Instance), _I, _I, _I, ctypes.POINTER(_P)],

@@ -256,6 +256,19 @@ class Solver:
     def release_reference(self):
         self._call("rrlbm_release_reference")
 
+    def graph_capture(self, nsteps):
+        """Capture nsteps fused steps into a HIP graph (after fused_pack)."""
+        self._call("rrlbm_graph_capture", int(nsteps))
+
+    def graph_launch(self, times=1):
+        self._call("rrlbm_graph_launch", int(times))
+
+    def graph_launch_timed(self, times=1):
+        """Replay the graph `times` times; returns the elapsed ms (HIP events)."""
+        ms = ctypes.c_double(0.0)
+        self._call("rrlbm_graph_launch_timed", int(times), ctypes.byref(ms))
+        return ms.value
+
     def autotune(self, trial_steps=20):
         """Pick the fastest population layout and launch shape for this
         configuration (before fused_pack); returns the choice as a dict."""

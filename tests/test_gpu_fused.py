@@ -224,3 +224,30 @@ def test_autotune_picks_a_shape_with_identical_results(rr):
     with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, **kw) as s:
         t2 = s.autotune(trial_steps=2)
         assert t2["candidates"] == 0 and t2["pop_group_rows"] == t["pop_group_rows"]
+
+
+@pytest.mark.parametrize("les", [0, 1])
+def test_graph_replays_match_eager_steps(rr, les):
+    """Fused steps captured into a HIP graph: three replays of 4 steps, with
+    eager work and synchronisation between them, give the state of 12 eager
+    steps bit for bit (FP32, with and without the lagged LES velocity
+    buffers); an odd capture length refuses its second replay."""
+    g = channel_state(oracle.D3Q27_CUDA, 64, 12, 8, u_tau=0.01, visc=0.002, seed=21, force=(2e-5, 0, 0))
+    kw = dict(storage=1, tau=0.5 + 3 * 0.002, force=(2e-5, 0, 0), wall_y=1, wall_rr=1, use_les=les)
+    f_eager, m_eager, d_eager = run_fused(rr, g, 13, **kw)
+    with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, **kw) as s:
+        upload_state(s, g)
+        s.fused_pack()
+        s.graph_capture(4)
+        ke = []
+        for rep in range(3):
+            s.graph_launch(1)
+            ke.append(s.fused_diag()["kinetic_energy"])
+        s.fused_unpack()
+        assert np.array_equal(s.download(rr.Field.F), f_eager)
+        assert ke[-1] == d_eager["kinetic_energy"]
+        s.graph_capture(3)
+        s.graph_launch(1)
+        with pytest.raises(rr.RRLBMError) as ei:
+            s.graph_launch(1)
+        assert ei.value.status == 3
