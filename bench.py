@@ -125,11 +125,35 @@ def run_ensemble(rrlbm, steps, warmup, storage=1):
     return cells, mlups, ke
 
 
+def small_ensemble(rrlbm, steps):
+    """64 small periodic TG boxes (N in {32, 48}, 4.3 M cells): the regime where
+    per-instance launches are launch-bound."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    inst = []
+    for _ in range(64):
+        n = int(rng.choice([32, 48]))
+        u0 = 0.05 / np.sqrt(3.0)
+        inst.append((n, n, n, 3.0 * u0 * (n / (2 * np.pi)) / 400.0 + 0.5, u0))
+    with rrlbm.Batch(inst, storage=1, keep_reference=False) as b:
+        cells = b.cells()
+        b.init_taylor_green_3d()
+        b.step(10)
+        b.step_sequential(10)
+        res = {"batched": b.time(0, steps), "sequential": b.time(1, steps)}
+        b.graph_capture(steps)
+        b.graph_launch(1)
+        res["graph"] = b.time(2, 1)
+    return cells, {k: cells * steps / (v / 1e3) / 1e6 for k, v in res.items()}
+
+
 def main_ensemble(args):
     import rrlbm
 
     rrlbm.load()
     cells, m, ke = run_ensemble(rrlbm, args.steps, args.warmup)
+    scells, sm = small_ensemble(rrlbm, 200)
     blu = 2 * 27 * 4
     best = max(m["graph"], m["batched"])
     line = {
@@ -143,20 +167,39 @@ def main_ensemble(args):
         "modes_mlups": {k: round(v, 1) for k, v in m.items()},
         "speedup_vs_sequential": round(best / m["sequential"], 3),
         "finite": bool(all(x == x for x in ke)),
+        "small_ensemble": {"workload": "64 periodic D3Q27 TG instances, N in {32, 48}, %d cells, FP32" % scells,
+                           "modes_mlups": {k: round(v, 1) for k, v in sm.items()},
+                           "speedup_vs_sequential": round(max(sm["graph"], sm["batched"]) / sm["sequential"], 3)},
     }
     print(json.dumps(line))
 
 
-def timed(s, steps, warmup):
-    """(MLUPS over the wall clock, mean kernel ms per step) of `steps` fused steps."""
+def timed(s, steps, warmup, tune=True, graph=False):
+    """(MLUPS over the wall clock, mean ms per step, tuning) of `steps` fused
+    steps, after autotuning (before the pack) when asked; graph=True replays a
+    captured graph of `graph_len` steps instead of eager launches."""
+    tuning = s.autotune(trial_steps=10) if tune else None
     s.fused_pack()
     s.fused_step(warmup)
     s.synchronize()
-    t0 = time.perf_counter()
-    ms = s.fused_step_timed(steps)
+    if graph:
+        glen = 100 if steps % 100 == 0 else steps
+        s.graph_capture(glen)
+        s.graph_launch(1)
+        s.synchronize()
+        t0 = time.perf_counter()
+        ms = s.graph_launch_timed(steps // glen)
+    else:
+        t0 = time.perf_counter()
+        ms = s.fused_step_timed(steps)
     s.synchronize()
     wall = time.perf_counter() - t0
-    return s.n * steps / wall / 1e6, ms / steps
+    return s.n * steps / wall / 1e6, ms / steps, tuning
+
+
+def shape_of(t):
+    return None if t is None else {k: t[k] for k in ("pop_group_rows", "pop_pad_bytes", "cells_per_thread", "xcd_remap",
+                                                     "persistent_wg_per_cu")}
 
 
 def line_for(metric, workload, mlups, ms, blu, dtype, steps, warmup, extra):
@@ -203,31 +246,42 @@ def main_configs(args):
     rrlbm.load()
     out = []
     # C4: channel LES-WALE 192x64x96, lagged gradient (one launch per step) and reference order
+    # (1.2 M cells: each step is ~30-50 us, so the steps run as graph replays)
     for st in (2, 1, 0):
         for lagged in (1, 0):
             s, meta = configs.channel_les(storage=st, les_lagged=lagged)
-            mlups, ms = timed(s, args.steps, args.warmup)
+            mlups, ms, tun = timed(s, args.steps, args.warmup, graph=True)
             s.close()
             log("C4 storage %d lagged %d: %.1f MLUPS" % (st, lagged, mlups))
             blu = 2 * 27 * STORAGE_BYTES[st] + 6 * (8 if st == 0 else 4) + (0 if lagged else 27 * STORAGE_BYTES[st])
             out.append(line_for("MLUPS, channel LES-WALE Re_tau=180 192x64x96 D3Q27", "C4 LES %s %s" % (
                 STORAGE_NAME[st], "lagged" if lagged else "reference-order"), mlups, ms, blu, STORAGE_NAME[st],
-                args.steps, args.warmup, {}))
+                args.steps, args.warmup, {"tuning": shape_of(tun), "launch": "graph"}))
+    # C3 grid with WALE (the DNS grid, LES on): the fused RR+WALE kernel at full size
+    for st in (2, 1):
+        s, meta = configs.channel(storage=st, les=True)
+        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        s.close()
+        log("C3 grid LES storage %d: %.1f MLUPS" % (st, mlups))
+        out.append(line_for("MLUPS, channel RR+WALE Re_tau=180 384x128x192 D3Q27", "C3 grid LES %s lagged"
+                            % STORAGE_NAME[st], mlups, ms, 2 * 27 * STORAGE_BYTES[st] + 24, STORAGE_NAME[st],
+                            args.steps, args.warmup, {"tuning": shape_of(tun)}))
     # C2: cavity 128^3 Re 1000, D3Q19 and D3Q27
     for q in (19, 27):
         for st in (1, 0):
             s, meta = configs.cavity(storage=st, npop=q)
-            mlups, ms = timed(s, args.steps, args.warmup)
+            mlups, ms, tun = timed(s, args.steps, args.warmup)
             s.close()
             log("C2 D3Q%d storage %d: %.1f MLUPS" % (q, st, mlups))
             out.append(line_for("MLUPS, lid-driven cavity 128^3 Re=1000", "C2 cavity D3Q%d %s" % (q, STORAGE_NAME[st]),
-                                mlups, ms, 2 * q * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup, {}))
+                                mlups, ms, 2 * q * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup,
+                                {"tuning": shape_of(tun)}))
     # C1: Taylor-Green 100x100x4 (40 000 cells: launch-bound), full 2 000 steps;
     # the CPU oracle runs the same 2 000 reference-order steps
     cpu = tg_cpu_baseline()
     for st in (0, 1):
         s, meta = configs.taylor_green_2d(storage=st)
-        mlups, ms = timed(s, 2000, 20)
+        mlups, ms, _ = timed(s, 2000, 20)
         # 40 000 cells are launch-bound: the same 2 000 steps as 20 replays of a
         # captured graph of 100 steps
         s.graph_capture(100)

@@ -386,18 +386,22 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     C omega;
     if constexpr (LES) {
       const bool wy = (a.walls & 2) != 0, wx = (a.walls & 1) != 0, wz = (a.walls & 4) != 0;
-      // xs[0] is x+1 and xs[2] is x-1 (the upwind sources for c = -1 and +1)
-      const uint32_t cxp = (z * a.ly + y) * a.lx + xs[0], cxm = (z * a.ly + y) * a.lx + xs[2];
-      const uint32_t cyp = (z * a.ly + ys[0]) * a.lx + x, cym = (z * a.ly + ys[2]) * a.lx + x;
-      const uint32_t czp = (zs[0] * a.ly + y) * a.lx + x, czm = (zs[2] * a.ly + y) * a.lx + x;
       const bool xlo = wx && x == 0, xhi = wx && x == a.lx - 1;
       const bool ylo = wy && y == 0, yhi = wy && y == a.ly - 1;
       const bool zlo = wz && z == 0, zhi = wz && z == a.lz - 1;
-      const void* u = a.u_in;
-      const Vel3<C> v0 = ld_v<C>(u, cell);
-      const Vel3<C> vxm = ld_v<C>(u, cxm), vxp = ld_v<C>(u, cxp);
-      const Vel3<C> vym = ld_v<C>(u, cym), vyp = ld_v<C>(u, cyp);
-      const Vel3<C> vzm = ld_v<C>(u, czm), vzp = ld_v<C>(u, czp);
+      // the five velocity rows of the stencil (scalar with ROWS) and the
+      // lane's x offsets; xs[0] is x+1 and xs[2] is x-1
+      constexpr uint32_t vb = sizeof(Vel3<C>);
+      auto vrow = [&](const void* base, uint32_t yy, uint32_t zz) {
+        return reinterpret_cast<const char*>(base) + (uint64_t)((zz * a.ly + yy) * a.lx) * vb;
+      };
+      auto vat = [](const char* row, uint32_t off) { return *reinterpret_cast<const Vel3<C>*>(row + off); };
+      const char* r00 = vrow(a.u_in, y, z);
+      const uint32_t o0 = x * vb, oxp = xs[0] * vb, oxm = xs[2] * vb;
+      const Vel3<C> v0 = vat(r00, o0);
+      const Vel3<C> vxm = vat(r00, oxm), vxp = vat(r00, oxp);
+      const Vel3<C> vym = vat(vrow(a.u_in, ys[2], z), o0), vyp = vat(vrow(a.u_in, ys[0], z), o0);
+      const Vel3<C> vzm = vat(vrow(a.u_in, y, zs[2]), o0), vzp = vat(vrow(a.u_in, y, zs[0]), o0);
       const C dudx = ddir<C>(v0.x, vxm.x, vxp.x, xlo, xhi);
       const C dvdx = ddir<C>(v0.y, vxm.y, vxp.y, xlo, xhi);
       const C dwdx = ddir<C>(v0.z, vxm.z, vxp.z, xlo, xhi);
@@ -409,7 +413,11 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       const C dwdz = ddir<C>(v0.z, vzm.z, vzp.z, zlo, zhi);
       const C tau_eff = C(a.tau) + wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
       omega = C(1) - recip(tau_eff);
-      if (a.write_u && active) st_v<C>(a.u_out, cell, q.ux, q.uy, q.uz);
+      if (a.write_u && active) {
+        uint32_t ow = o0;
+        asm volatile("" : "+v"(ow));  // 32-bit lane offset in this block (scalar-base store)
+        *reinterpret_cast<Vel3<C>*>(const_cast<char*>(vrow(a.u_out, y, z)) + ow) = Vel3<C>{q.ux, q.uy, q.uz};
+      }
     }
     if constexpr (!LES) omega = C(a.omega);
     C out[Q];
