@@ -239,7 +239,7 @@ def log(msg):
 
 
 def main_configs(args):
-    """C1 Taylor-Green, C2 cavity, C4 LES channel: one JSON line each."""
+    """C4 LES channel, C3-grid LES and BGK, C2 cavity, C1 Taylor-Green: one JSON line each."""
     import rrlbm
     from rrlbm import configs
 
@@ -266,6 +266,15 @@ def main_configs(args):
         out.append(line_for("MLUPS, channel RR+WALE Re_tau=180 384x128x192 D3Q27", "C3 grid LES %s lagged"
                             % STORAGE_NAME[st], mlups, ms, 2 * 27 * STORAGE_BYTES[st] + 24, STORAGE_NAME[st],
                             args.steps, args.warmup, {"tuning": shape_of(tun)}))
+    # C3 grid under the BGK baseline operator (A/B against the RR step on the same grid)
+    for st in (1, 2):
+        s, meta = configs.channel_bgk(storage=st)
+        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        s.close()
+        log("C3 grid BGK storage %d: %.1f MLUPS" % (st, mlups))
+        out.append(line_for("MLUPS, channel BGK Re_tau=180 384x128x192 D3Q27", "C3 grid BGK %s" % STORAGE_NAME[st],
+                            mlups, ms, 2 * 27 * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup,
+                            {"tuning": shape_of(tun)}))
     # C2: cavity 128^3 Re 1000, D3Q19 and D3Q27
     for q in (19, 27):
         for st in (1, 0):

@@ -49,6 +49,14 @@ typedef enum rrlbm_status {
     RRLBM_E_UNSUPPORTED = 6  /* valid request this build does not provide */
 } rrlbm_status;
 
+/* collision operator: the RR operator of the channel solver (RR27V33_code_DNS_LES/
+ * collision.cu) or the BGK baseline of the reference's BGK solver (BGK/collision.cu:
+ * incompressible second-order equilibrium, u = j + F/2 without division by rho,
+ * BGK/macrovar.cu:27; D3Q27, no LES, no RR wall regularisation).  The library's
+ * reference-layout fields keep their convention under BGK: f is unshifted and rho
+ * is 1 + the BGK solver's density deviation (that solver stores f - w and rho - 1). */
+typedef enum rrlbm_collision { RRLBM_COLLISION_RR = 0, RRLBM_COLLISION_BGK = 1 } rrlbm_collision;
+
 /* storage of the fused path's populations */
 typedef enum rrlbm_storage { RRLBM_STORE_F64 = 0, RRLBM_STORE_F32 = 1, RRLBM_STORE_F16 = 2 } rrlbm_storage;
 
@@ -112,7 +120,8 @@ typedef struct rrlbm_params {
     int32_t persistent_wg_per_cu; /* fused launches: 0 = one workgroup per block of cells;
                                    k > 0 = a persistent grid of k workgroups per compute unit
                                    striding over the blocks */
-    int32_t reserved[2];
+    int32_t collision_model;    /* rrlbm_collision (default RR) */
+    int32_t reserved[1];
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;
@@ -177,16 +186,19 @@ int rrlbm_bb(rrlbm_ctx* ctx);
 /* The D2D copy f_temp -> f of main.cu:64, done as a buffer swap: afterwards
  * f holds the streamed populations and f_temp the post-collision ones. */
 int rrlbm_swap_f(rrlbm_ctx* ctx);
-/* macrovar, macrovar.cu:4-31. */
+/* macrovar, macrovar.cu:4-31 (BGK model: BGK/macrovar.cu:4-31, u = j + F/2). */
 int rrlbm_macrovar(rrlbm_ctx* ctx);
+/* collision_BGK, BGK/collision.cu:3-33 (in place; needs collision_model = BGK). */
+int rrlbm_collision_bgk(rrlbm_ctx* ctx);
 /* RRBC, BC.cu:4-126 (needs rrlbm_swap_f to have run: re-imposed bounce-back
  * values at multi-face cells come from f_temp). */
 int rrlbm_rrbc(rrlbm_ctx* ctx);
 /* macrovar_BN, macrovar.cu:33-62. */
 int rrlbm_macrovar_bn(rrlbm_ctx* ctx);
 /* nsteps of main.cu:50-84 with the reference kernels: [LES_WALE] ->
- * collision_RR -> streaming -> BB -> copy -> macrovar -> [RRBC -> macrovar_BN].
- * The force fields are used as they are. */
+ * collision_RR -> streaming -> BB -> copy -> macrovar -> [RRBC -> macrovar_BN]
+ * (BGK model: BGK/main.cu:49-70, collision_BGK -> streaming -> BB -> copy ->
+ * macrovar).  The force fields are used as they are. */
 int rrlbm_step_reference(rrlbm_ctx* ctx, int nsteps);
 
 /* ------------------------------------------------------- initialisation */

@@ -184,6 +184,23 @@ class Fields:
                                 _p(self.fz), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz),
                                 ctypes.c_double(tau), _I(int(stale_fbar)))
 
+    def collision_bgk(self, tau):
+        """collision_BGK, BGK/collision.cu:3-33 (D3Q27, shifted populations)."""
+        assert self.kind == D3Q27_CUDA
+        load().orc_collision_bgk(*self.dims, _p(self.f), _p(self.fx), _p(self.fy), _p(self.fz), _p(self.rho),
+                                 _p(self.ux), _p(self.uy), _p(self.uz), ctypes.c_double(tau))
+
+    def macrovar_bgk(self):
+        load().orc_macrovar_bgk(*self.dims, _p(self.f), _p(self.fx), _p(self.fy), _p(self.fz), _p(self.rho),
+                                _p(self.ux), _p(self.uy), _p(self.uz))
+
+    def step_bgk(self, nsteps, tau, walls=(0, 1, 0)):
+        """BGK/main.cu:49-70 order, nsteps times."""
+        assert self.kind == D3Q27_CUDA
+        load().orc_step_bgk(*self.dims, _I(nsteps), _p(self.f), _p(self.ftmp), _p(self.fx), _p(self.fy),
+                            _p(self.fz), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz), ctypes.c_double(tau),
+                            *[_I(w) for w in walls])
+
     def diag(self):
         out = np.zeros(9)
         load().orc_diag(ctypes.c_long(self.n), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz), _p(out))

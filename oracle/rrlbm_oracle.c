@@ -33,6 +33,11 @@
  *                                Taylor_Green/RRBMa0.01L100.txt (20 000 steps)
  *   orc_step_laminar          -> committed FP64 fields Laminar/output/uy*.dat
  *                                (5 000 and 10 000 steps, 1e-14 relative)
+ *   orc_collision_bgk,        -> mass conservation, momentum change = F,
+ *   orc_macrovar_bgk             the rest state as a fixed point, the
+ *                                second-moment relaxation rate 1 - 1/tau
+ *                                ("parity unpinned" beyond that: the BGK
+ *                                solver's repository has no committed fields)
  *   orc_forcing_gp, orc_init_arrays -> closed-form spot values of the cited
  *                                formulas ("parity unpinned" beyond that: no
  *                                committed reference output exists)
@@ -596,6 +601,74 @@ void orc_step_laminar(int lx, int ly, int lz, int nsteps, double* f, double* ftm
                 }
         }
         orc_macrovar(ORC_D3Q27_CUDA, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 0, 0, 0, 0);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* The BGK baseline solver (BGK/): incompressible D3Q27 BGK on shifted  */
+/* populations, density deviation rho and u = j + F/2 (no division)     */
+/* ------------------------------------------------------------------ */
+/* collision_BGK, BGK/collision.cu:3-33 (in place): with eu = c.u/cs^2 and
+ * uv = |u|^2/cs^2, feq = w (rho + eu + (eu^2 - uv)/2), feq0 the same with
+ * rho = 1, f += -(f - feq)/tau + (1 - 1/(2 tau)) (c - u).F / cs^2 feq0. */
+void orc_collision_bgk(int lx, int ly, int lz, double* f, const double* fx, const double* fy, const double* fz,
+                       const double* rho, const double* ux, const double* uy, const double* uz, double tau) {
+    orc_lat L;
+    lat_build(ORC_D3Q27_CUDA, &L);
+    const long N = (long)lx * ly * lz;
+    const double RT = 1.0 / 3.0;
+#pragma omp parallel for schedule(static)
+    for (long idx = 0; idx < N; ++idx) {
+        const double u = ux[idx], v = uy[idx], w = uz[idx];
+        const double G3 = u * fx[idx] + v * fy[idx] + w * fz[idx];
+        for (int ip = 0; ip < 27; ++ip) {
+            const double eu = (L.cx[ip] * u + L.cy[ip] * v + L.cz[ip] * w) / RT;
+            const double uv = (u * u + v * v + w * w) / RT;
+            const double feq = L.w[ip] * (rho[idx] + eu + 0.5 * (eu * eu - uv));
+            const double G1 = L.cx[ip] * fx[idx] + L.cy[ip] * fy[idx] + L.cz[ip] * fz[idx];
+            double fchange = -(f[ip * N + idx] - feq) / tau;
+            const double feq0 = L.w[ip] * (1.0 + eu + 0.5 * (eu * eu - uv));
+            fchange += (1.0 - 0.5 / tau) * (G1 - G3) / RT * feq0;
+            f[ip * N + idx] += fchange;
+        }
+    }
+}
+
+/* macrovar of the BGK solver, BGK/macrovar.cu:4-31: rho = sum f (the density
+ * deviation), u = sum c f + F/2. */
+void orc_macrovar_bgk(int lx, int ly, int lz, const double* f, const double* fx, const double* fy, const double* fz,
+                      double* rho, double* ux, double* uy, double* uz) {
+    orc_lat L;
+    lat_build(ORC_D3Q27_CUDA, &L);
+    const long N = (long)lx * ly * lz;
+#pragma omp parallel for schedule(static)
+    for (long idx = 0; idx < N; ++idx) {
+        double r = 0, jx = 0, jy = 0, jz = 0;
+        for (int ip = 0; ip < 27; ++ip) {
+            const double v = f[ip * N + idx];
+            r += v; jx += L.cx[ip] * v; jy += L.cy[ip] * v; jz += L.cz[ip] * v;
+        }
+        rho[idx] = r;
+        ux[idx] = jx + 0.5 * fx[idx];
+        uy[idx] = jy + 0.5 * fy[idx];
+        uz[idx] = jz + 0.5 * fz[idx];
+    }
+}
+
+/* One step of the BGK solver's loop, BGK/main.cu:49-70 (the force arrays as
+ * they are): collision_BGK -> streaming (periodic push, BGK/streaming.cu:3-29)
+ * -> streaming2 (half-way bounce-back at the y walls, :31-63) -> copy ->
+ * macrovar.  walls: as orc_bb (the reference has y walls). */
+void orc_step_bgk(int lx, int ly, int lz, int nsteps, double* f, double* ftmp, const double* fx, const double* fy,
+                  const double* fz, double* rho, double* ux, double* uy, double* uz, double tau, int wall_x,
+                  int wall_y, int wall_z) {
+    const long N = (long)lx * ly * lz;
+    for (int s = 0; s < nsteps; ++s) {
+        orc_collision_bgk(lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, tau);
+        orc_streaming(ORC_D3Q27_CUDA, lx, ly, lz, f, ftmp);
+        orc_bb(ORC_D3Q27_CUDA, lx, ly, lz, f, ftmp, wall_x, wall_y, wall_z, -1, NULL);
+        for (long i = 0; i < 27 * N; ++i) { double t = f[i]; f[i] = ftmp[i]; ftmp[i] = t; }
+        orc_macrovar_bgk(lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz);
     }
 }
 

@@ -131,6 +131,8 @@ struct FusedArgs {
 // Host launchers (kernels_fused.hip)
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s);
+// The BGK baseline operator (D3Q27, no LES): step, finish and diagnostics modes.
+hipError_t launch_fused_bgk(int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list

@@ -260,7 +260,9 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
 // then wavefront-uniform (scalar registers): each population's source address
 // is a scalar row base plus one of three per-lane x offsets, instead of a
 // per-lane three-term sum.  Otherwise (any lx) every lane computes its own.
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS>
+// BGK: the baseline BGK operator (bgk_post) instead of the RR collision, with
+// the BGK solver's velocity u = j + F/2 (D3Q27, no LES, no wall regularisation).
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, bool BGK = false>
 RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
   using T = typename StoreT<S>::T;
@@ -334,7 +336,16 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   Moments<C> m;
   if constexpr (Q == 27) m = moments_q27<C>(h);
   else m = moments_generic<L, C>(h);
-  const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
+  Macro<C> q;
+  if constexpr (BGK) {
+    q.dr = m.dr;
+    q.rho = C(1) + m.dr;
+    q.ux = m.jx + C(0.5) * Fx;
+    q.uy = m.jy + C(0.5) * Fy;
+    q.uz = m.jz + C(0.5) * Fz;
+  } else {
+    q = macro_of<C, FM>(m, Fx, Fy, Fz);
+  }
 
   if constexpr (MODE == kMacro) {
     if (active) st_v<C>(a.u_out, cell, q.ux, q.uy, q.uz);
@@ -421,7 +432,14 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     }
     if constexpr (!LES) omega = C(a.omega);
     C out[Q];
-    if constexpr (Q == 27) {
+    if constexpr (BGK) {
+      const C inv_tau = C(1) - omega;
+      const C fcoef = C(1) - C(0.5) * inv_tau;
+      if constexpr (S == kF16)
+        bgk_post<L, C, false>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(a.f16_scale), out);
+      else
+        bgk_post<L, C, true>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(1), out);
+    } else if constexpr (Q == 27) {
       const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
       if constexpr (S == kF16)
         rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
@@ -455,21 +473,21 @@ RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
 }
 
 // One workgroup per block of 256 cells (grid = a.tiles).
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS>
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, bool BGK = false>
 __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
   const uint32_t b = block_of(a, blockIdx.x);
-  if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS>(a, b);
+  if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS, BGK>(a, b);
 }
 
 // Persistent grid (step mode): fewer workgroups than blocks, each striding
 // over the block sequence; every block is computed once per launch.  A
 // separate kernel, because the loop raises the register count of the body
 // (loop-invariant addresses stay live), which one-block workgroups avoid.
-template <int S, class L, bool WALLS, bool LES, int FM, bool ROWS>
+template <int S, class L, bool WALLS, bool LES, int FM, bool ROWS, bool BGK = false>
 __global__ __launch_bounds__(kBlock) void fused_kernel_persistent(FusedArgs a) {
   for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
     const uint32_t b = block_of(a, t);
-    if (b < a.nblocks) fused_body<S, L, WALLS, LES, kStep, FM, ROWS>(a, b);
+    if (b < a.nblocks) fused_body<S, L, WALLS, LES, kStep, FM, ROWS, BGK>(a, b);
   }
 }
 
@@ -697,6 +715,41 @@ hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32
     case kF64: return walls ? launch_pair_force<kF64, true>(fm, a, grid, s) : launch_pair_force<kF64, false>(fm, a, grid, s);
     case kF32: return walls ? launch_pair_force<kF32, true>(fm, a, grid, s) : launch_pair_force<kF32, false>(fm, a, grid, s);
     case kF16: return walls ? launch_pair_force<kF16, true>(fm, a, grid, s) : launch_pair_force<kF16, false>(fm, a, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// The BGK baseline: D3Q27, no LES, general force (step, finish and
+// diagnostics passes).
+template <int S, bool W, bool R>
+static hipError_t launch_bgk_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  if (mode == kStep && grid < a.tiles) {
+    hipLaunchKernelGGL((fused_kernel_persistent<S, D3Q27, W, false, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0,
+                       s, a);
+    return hipGetLastError();
+  }
+  switch (mode) {
+    case kStep: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kStep, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kFinish: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kFinish, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kDiag: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kDiag, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_bgk_storage(bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  const bool rows = a.lx % 64u == 0;
+  if (walls)
+    return rows ? launch_bgk_mode<S, true, true>(mode, a, grid, s) : launch_bgk_mode<S, true, false>(mode, a, grid, s);
+  return rows ? launch_bgk_mode<S, false, true>(mode, a, grid, s) : launch_bgk_mode<S, false, false>(mode, a, grid, s);
+}
+
+hipError_t launch_fused_bgk(int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  switch (storage) {
+    case kF64: return launch_bgk_storage<kF64>(walls, mode, a, grid, s);
+    case kF32: return launch_bgk_storage<kF32>(walls, mode, a, grid, s);
+    case kF16: return launch_bgk_storage<kF16>(walls, mode, a, grid, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -70,6 +70,26 @@ __global__ __launch_bounds__(kBlock) void k_collision(RefFields r, Grid g, doubl
   });
 }
 
+// collision_BGK (BGK/collision.cu:3-33), in place, in this library's field
+// convention (f unshifted, rho = 1 + the BGK solver's density deviation):
+// the BGK solver's shifted populations are f - w.
+__global__ __launch_bounds__(kBlock) void k_collision_bgk(RefFields r, Grid g, double tau) {
+  const uint32_t cell = tid();
+  if (cell >= g.n) return;
+  using L = D3Q27;
+  double h[27], out[27];
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    h[i] = r.f[(size_t)i * g.n + cell] - L::w(i);
+  });
+  bgk_post<L, double, true>(h, r.rho[cell] - 1.0, r.ux[cell], r.uy[cell], r.uz[cell], r.fx[cell], r.fy[cell],
+                            r.fz[cell], 1.0 / tau, 1.0 - 0.5 / tau, 1.0, out);
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    r.f[(size_t)i * g.n + cell] = out[i] + L::w(i);
+  });
+}
+
 // streaming (streaming.cu:3-24): push into ftmp, periodic in x, y, z.
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_streaming(const double* f, double* ftmp, Grid g) {
@@ -113,8 +133,9 @@ RRLBM_INLINE __device__ bool on_any_wall(const Cell3& c, const Grid& g) {
 }
 
 // macrovar (macrovar.cu:4-31); wall_only: macrovar_BN (macrovar.cu:33-62).
+// bgk: the BGK solver's velocity u = j + F/2 (BGK/macrovar.cu:4-31).
 template <class L>
-__global__ __launch_bounds__(kBlock) void k_macrovar(RefFields r, Grid g, int wall_only) {
+__global__ __launch_bounds__(kBlock) void k_macrovar(RefFields r, Grid g, int wall_only, int bgk) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
   if (wall_only && !on_any_wall(coords(cell, g), g)) return;
@@ -128,9 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_macrovar(RefFields r, Grid g, int wa
     if constexpr (L::cz(i) != 0) jz += L::cz(i) * v;
   });
   r.rho[cell] = rho;
-  r.ux[cell] = (jx + 0.5 * r.fx[cell]) / rho;
-  r.uy[cell] = (jy + 0.5 * r.fy[cell]) / rho;
-  r.uz[cell] = (jz + 0.5 * r.fz[cell]) / rho;
+  const double inv = bgk ? 1.0 : 1.0 / rho;
+  r.ux[cell] = (jx + 0.5 * r.fx[cell]) * inv;
+  r.uy[cell] = (jy + 0.5 * r.fy[cell]) * inv;
+  r.uz[cell] = (jz + 0.5 * r.fz[cell]) * inv;
 }
 
 // RRBC (BC.cu:4-126): at wall cells replace the wall-incoming populations by
@@ -290,12 +312,24 @@ __global__ __launch_bounds__(kBlock) void k_init_arrays(RefFields r, Grid g, dou
 }
 
 // Populations = RR equilibrium of the rho, u fields.
+// bgk: the BGK solver's second-order incompressible equilibrium
+// w (rho + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2).
 template <class L>
-__global__ __launch_bounds__(kBlock) void k_init_equilibrium(RefFields r, Grid g) {
+__global__ __launch_bounds__(kBlock) void k_init_equilibrium(RefFields r, Grid g, int bgk) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
   double out[L::Q];
-  feq_shifted<L, double>(r.rho[cell], r.ux[cell], r.uy[cell], r.uz[cell], out);
+  if (bgk) {
+    const double ux = r.ux[cell], uy = r.uy[cell], uz = r.uz[cell];
+    const double uu = 1.5 * (ux * ux + uy * uy + uz * uz);
+    static_for<L::Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double cu = L::cx(i) * ux + L::cy(i) * uy + L::cz(i) * uz;
+      out[i] = L::w(i) * (r.rho[cell] - 1.0 + 3.0 * cu + 4.5 * cu * cu - uu);
+    });
+  } else {
+    feq_shifted<L, double>(r.rho[cell], r.ux[cell], r.uy[cell], r.uz[cell], out);
+  }
   static_for<L::Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     r.f[(size_t)i * g.n + cell] = out[i] + L::w(i);
@@ -362,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(RefFields r, Grid g, Ini
 // the fused kernel's storage.
 template <int S, class L>
 __global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst, PopLayout lay, double tau, int use_les,
-                                                 double fx, double fy, double fz, double f16_scale) {
+                                                 double fx, double fy, double fz, double f16_scale, int bgk) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
   constexpr int Q = L::Q;
@@ -377,9 +411,14 @@ __global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst,
   const double omega = 1.0 - 1.0 / (tau + (use_les ? r.taut[cell] : 0.0));
   double out[Q];
   if constexpr (Q == 27) {
-    const Moments<double> m = moments_q27<double>(h);
-    const NeqTensor<double> N = neq_q27<double>(m, q, fx, fy, fz);
-    rr_post_q27<double, true>(q, N, fx, fy, fz, omega, 1.0, 1.0, out);
+    if (bgk) {
+      bgk_post<L, double, true>(h, q.dr, q.ux, q.uy, q.uz, fx, fy, fz, 1.0 - omega, 1.0 - 0.5 * (1.0 - omega), 1.0,
+                                out);
+    } else {
+      const Moments<double> m = moments_q27<double>(h);
+      const NeqTensor<double> N = neq_q27<double>(m, q, fx, fy, fz);
+      rr_post_q27<double, true>(q, N, fx, fy, fz, omega, 1.0, 1.0, out);
+    }
   } else {
     rr_post_generic<L, double>(h, q, fx, fy, fz, omega, 1.0, out);
   }
@@ -506,9 +545,12 @@ hipError_t ref_bb(int q, const double* f, double* ftmp, const Grid& g, hipStream
   if (q == 27) RRLBM_LAUNCH(k_bb<D3Q27>, g, s, f, ftmp, g);
   RRLBM_LAUNCH(k_bb<D3Q19>, g, s, f, ftmp, g);
 }
-hipError_t ref_macrovar(int q, const RefFields& r, const Grid& g, int wall_only, hipStream_t s) {
-  if (q == 27) RRLBM_LAUNCH(k_macrovar<D3Q27>, g, s, r, g, wall_only);
-  RRLBM_LAUNCH(k_macrovar<D3Q19>, g, s, r, g, wall_only);
+hipError_t ref_macrovar(int q, const RefFields& r, const Grid& g, int wall_only, hipStream_t s, int bgk) {
+  if (q == 27) RRLBM_LAUNCH(k_macrovar<D3Q27>, g, s, r, g, wall_only, bgk);
+  RRLBM_LAUNCH(k_macrovar<D3Q19>, g, s, r, g, wall_only, bgk);
+}
+hipError_t ref_collision_bgk(const RefFields& r, const Grid& g, double tau, hipStream_t s) {
+  RRLBM_LAUNCH(k_collision_bgk, g, s, r, g, tau);
 }
 hipError_t ref_rrbc(int q, const RefFields& r, const double* fpost, const Grid& g, hipStream_t s) {
   if (q == 27) RRLBM_LAUNCH(k_rrbc<D3Q27>, g, s, r, fpost, g);
@@ -529,16 +571,16 @@ hipError_t ref_init_arrays(int q, const RefFields& r, const Grid& g, double usta
   if (q == 27) RRLBM_LAUNCH(k_init_arrays<D3Q27>, g, s, r, g, ustar, ystar, fx);
   RRLBM_LAUNCH(k_init_arrays<D3Q19>, g, s, r, g, ustar, ystar, fx);
 }
-hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStream_t s) {
-  if (q == 27) RRLBM_LAUNCH(k_init_equilibrium<D3Q27>, g, s, r, g);
-  RRLBM_LAUNCH(k_init_equilibrium<D3Q19>, g, s, r, g);
+hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStream_t s, int bgk) {
+  if (q == 27) RRLBM_LAUNCH(k_init_equilibrium<D3Q27>, g, s, r, g, bgk);
+  RRLBM_LAUNCH(k_init_equilibrium<D3Q19>, g, s, r, g, bgk);
 }
 hipError_t ref_init_fields(const RefFields& r, const Grid& g, const InitSpec& spec, hipStream_t s) {
   RRLBM_LAUNCH(k_init_fields, g, s, r, g, spec);
 }
 hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, const PopLayout& lay, double tau,
-                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s) {
-#define RRLBM_PACK(S, LT) RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, lay, tau, use_les, fx, fy, fz, f16_scale)
+                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk) {
+#define RRLBM_PACK(S, LT) RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, lay, tau, use_les, fx, fy, fz, f16_scale, bgk)
   if (q == 27) {
     if (storage == kF64) RRLBM_PACK(kF64, D3Q27);
     if (storage == kF32) RRLBM_PACK(kF32, D3Q27);

@@ -38,7 +38,10 @@ struct InitSpec {
 hipError_t ref_collision(int q, const RefFields& r, const Grid& g, double tau, int use_les, hipStream_t s);
 hipError_t ref_streaming(int q, const double* f, double* ftmp, const Grid& g, hipStream_t s);
 hipError_t ref_bb(int q, const double* f, double* ftmp, const Grid& g, hipStream_t s);
-hipError_t ref_macrovar(int q, const RefFields& r, const Grid& g, int wall_only, hipStream_t s);
+// bgk: the BGK solver's velocity u = j + F/2 (BGK/macrovar.cu:4-31)
+hipError_t ref_macrovar(int q, const RefFields& r, const Grid& g, int wall_only, hipStream_t s, int bgk = 0);
+// collision_BGK (BGK/collision.cu:3-33), D3Q27
+hipError_t ref_collision_bgk(const RefFields& r, const Grid& g, double tau, hipStream_t s);
 hipError_t ref_rrbc(int q, const RefFields& r, const double* fpost, const Grid& g, hipStream_t s);
 hipError_t ref_les_wale(const RefFields& r, const Grid& g, double cw, hipStream_t s);
 hipError_t ref_forcing(const RefFields& r, const Grid& g, double fx, double fy, double fz, hipStream_t s);
@@ -46,10 +49,10 @@ hipError_t ref_forcing_gp(const RefFields& r, const Grid& g, double force_in_x, 
                           hipStream_t s);
 hipError_t ref_init_arrays(int q, const RefFields& r, const Grid& g, double ustar, double ystar, double fx,
                            hipStream_t s);
-hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStream_t s);
+hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStream_t s, int bgk = 0);
 hipError_t ref_init_fields(const RefFields& r, const Grid& g, const InitSpec& spec, hipStream_t s);
 hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, const PopLayout& lay, double tau,
-                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s);
+                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk = 0);
 hipError_t ref_copy_u(int storage, const RefFields& r, const Grid& g, void* u, hipStream_t s);
 hipError_t ref_plane_profiles(const RefFields& r, const Grid& g, int with_taut, double* out, hipStream_t s);
 hipError_t ref_diag(const RefFields& r, const Grid& g, double* partials, hipStream_t s);

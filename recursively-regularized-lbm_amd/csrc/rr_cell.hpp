@@ -435,6 +435,36 @@ RRLBM_HD RRLBM_INLINE void rr_post_generic(const C (&h)[L::Q], const Macro<C>& q
   });
 }
 
+// BGK baseline operator (BGK/collision.cu:3-33), the incompressible form of
+// the reference's BGK solver on shifted populations h = f - w, with the density
+// deviation dr = sum h and u = j + F/2 (no division by rho, BGK/macrovar.cu:27):
+//   feq_i - w_i = w_i (dr + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2)
+//   out_i = h_i - (h_i - (feq_i - w_i))/tau + (1 - 1/(2 tau)) 3 (c_i - u).F feq0_i,
+//   feq0_i = w_i (1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2),
+// each written times scale_i (w_i / w_i = 1 for shifted values, S / w_i for
+// the per-direction-scaled FP16 store).
+template <class L, typename C, bool per_direction_w>
+RRLBM_HD RRLBM_INLINE void bgk_post(const C (&h)[L::Q], C dr, C ux, C uy, C uz, C Fx, C Fy, C Fz, C inv_tau,
+                                    C fcoef, C scale, C (&out)[L::Q]) {
+  const C uu = C(1.5) * (ux * ux + uy * uy + uz * uz);
+  const C uF = ux * Fx + uy * Fy + uz * Fz;
+  static_for<L::Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int x = L::cx(i), y = L::cy(i), z = L::cz(i);
+    const C cu = C(x) * ux + C(y) * uy + C(z) * uz;
+    const C poly = C(3) * cu + C(4.5) * cu * cu - uu;          // 3 c.u + 9/2 (c.u)^2 - 3/2 u^2
+    const C cF = C(x) * Fx + C(y) * Fy + C(z) * Fz;
+    const C g = C(L::w(i));
+    const C feq = g * (dr + poly);                                // feq_i - w_i
+    const C frc = fcoef * C(3) * (cF - uF) * g * (C(1) + poly);
+    const C v = h[i] - (h[i] - feq) * inv_tau + frc;
+    if constexpr (per_direction_w)
+      out[i] = v;
+    else
+      out[i] = v * (scale * C(1.0 / L::w(i)));
+  });
+}
+
 // Shifted RR equilibrium of (rho, u): feq_i - w_i (collision.cu:57-69).
 template <class L, typename C>
 RRLBM_HD RRLBM_INLINE void feq_shifted(C rho, C ux, C uy, C uz, C (&out)[L::Q]) {

@@ -125,6 +125,7 @@ double* field_ptr(rrlbm_ctx* c, rrlbm_field f, size_t* count) {
 }
 
 bool has_walls(const rrlbm_params& p) { return p.wall_x || p.wall_y || p.wall_z; }
+bool is_bgk(const rrlbm_params& p) { return p.collision_model == RRLBM_COLLISION_BGK; }
 
 void set_blocks(FusedArgs& a, uint32_t nblocks, int remap) {
   a.nblocks = nblocks;
@@ -209,7 +210,9 @@ uint32_t fused_grid(const rrlbm_ctx* c, const FusedArgs& a) {
 
 int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
   const bool les = c->p.use_les && mode == kStep;
-  hipError_t e = launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a), c->stream);
+  hipError_t e = is_bgk(c->p) ? launch_fused_bgk(c->p.storage, has_walls(c->p), mode, a, fused_grid(c, a), c->stream)
+                              : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a),
+                                             c->stream);
   if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
   return RRLBM_OK;
 }
@@ -217,7 +220,7 @@ int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
 // Two cells per thread for the plain D3Q27 step only when requested: on
 // MI355X one cell per thread is faster for every storage (higher occupancy).
 bool use_pair(const rrlbm_ctx* c) {
-  if (c->q != 27 || c->p.use_les) return false;
+  if (c->q != 27 || c->p.use_les || is_bgk(c->p)) return false;
   return c->p.cells_per_thread == 2;
 }
 
@@ -401,6 +404,10 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   // cache lines in one L2 (measured: FP16 channel reads 1.06x instead of 1.23x
   // the algorithmic bytes); small grids keep the linear order
   if (p.xcd_remap == -1) p.xcd_remap = (n64 + kBlock - 1) / kBlock >= 2048 ? 32 : 0;
+  if (p.collision_model != RRLBM_COLLISION_RR && p.collision_model != RRLBM_COLLISION_BGK)
+    return fail(nullptr, RRLBM_E_INVALID, "collision_model must be 0 (RR) or 1 (BGK)");
+  if (p.collision_model == RRLBM_COLLISION_BGK && (p.npop != 27 || p.use_les || p.wall_rr))
+    return fail(nullptr, RRLBM_E_UNSUPPORTED, "the BGK baseline is D3Q27 without LES and without RR wall regularisation");
   if (p.cells_per_thread < 0 || p.cells_per_thread > 2)
     return fail(nullptr, RRLBM_E_INVALID, "cells_per_thread must be 0, 1 or 2");
   if (p.persistent_wg_per_cu < 0 || p.persistent_wg_per_cu > 64)
@@ -463,7 +470,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   // rho = 1 and the uniform force, as initialize() leaves them
   e = ref_forcing(c->ref, c->grid, p.force[0], p.force[1], p.force[2], c->stream);
   if (e == hipSuccess) e = ref_init_fields(c->ref, c->grid, InitSpec{kInitRest, 0, 0, 0, 0, 0, 0}, c->stream);
-  if (e == hipSuccess) e = ref_init_equilibrium(c->q, c->ref, c->grid, c->stream);
+  if (e == hipSuccess) e = ref_init_equilibrium(c->q, c->ref, c->grid, c->stream, is_bgk(c->p));
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return bail(fail(c, RRLBM_E_HIP, "initial fill failed: %s", hipGetErrorString(e)));
   *out = c;
@@ -594,7 +601,14 @@ int rrlbm_swap_f(rrlbm_ctx* c) {
 
 int rrlbm_macrovar(rrlbm_ctx* c) {
   NEED_REF(c);
-  CK(c, ref_macrovar(c->q, c->ref, c->grid, 0, c->stream));
+  CK(c, ref_macrovar(c->q, c->ref, c->grid, 0, c->stream, is_bgk(c->p)));
+  return RRLBM_OK;
+}
+
+int rrlbm_collision_bgk(rrlbm_ctx* c) {
+  NEED_REF(c);
+  if (!is_bgk(c->p)) return fail(c, RRLBM_E_STATE, "collision_bgk needs params.collision_model = RRLBM_COLLISION_BGK");
+  CK(c, ref_collision_bgk(c->ref, c->grid, c->p.tau, c->stream));
   return RRLBM_OK;
 }
 
@@ -606,7 +620,7 @@ int rrlbm_rrbc(rrlbm_ctx* c) {
 
 int rrlbm_macrovar_bn(rrlbm_ctx* c) {
   NEED_REF(c);
-  if (c->grid.walls) CK(c, ref_macrovar(c->q, c->ref, c->grid, 1, c->stream));
+  if (c->grid.walls) CK(c, ref_macrovar(c->q, c->ref, c->grid, 1, c->stream, is_bgk(c->p)));
   return RRLBM_OK;
 }
 
@@ -616,7 +630,8 @@ int rrlbm_step_reference(rrlbm_ctx* c, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     int rc;
     if (c->p.use_les && (rc = rrlbm_les_wale(c))) return rc;
-    if ((rc = rrlbm_collision_rr(c))) return rc;
+    // BGK/main.cu:49-70: collision_BGK -> streaming -> streaming2 (BB) -> copy -> macrovar
+    if ((rc = is_bgk(c->p) ? rrlbm_collision_bgk(c) : rrlbm_collision_rr(c))) return rc;
     if ((rc = rrlbm_streaming(c))) return rc;
     if ((rc = rrlbm_bb(c))) return rc;
     if ((rc = rrlbm_swap_f(c))) return rc;
@@ -632,7 +647,7 @@ int rrlbm_step_reference(rrlbm_ctx* c, int nsteps) {
 
 static int init_common(rrlbm_ctx* c, const InitSpec& s) {
   CK(c, ref_init_fields(c->ref, c->grid, s, c->stream));
-  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream));
+  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream, is_bgk(c->p)));
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   c->packed = false;
@@ -673,7 +688,7 @@ int rrlbm_init_channel(rrlbm_ctx* c, double u_tau, double visc, uint64_t seed, d
 
 int rrlbm_init_equilibrium(rrlbm_ctx* c) {
   NEED_REF(c);
-  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream));
+  CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream, is_bgk(c->p)));
   c->packed = false;
   return RRLBM_OK;
 }
@@ -682,7 +697,7 @@ int rrlbm_fused_pack(rrlbm_ctx* c) {
   NEED_REF(c);
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
   CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->lay, c->p.tau, c->p.use_les, c->p.force[0],
-                 c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream));
+                 c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream, is_bgk(c->p)));
   if (c->p.use_les) CK(c, ref_copy_u(c->p.storage, c->ref, c->grid, c->u_a, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   c->packed = true;
@@ -872,8 +887,9 @@ int force_mode_key(const rrlbm_params& p) {
 
 std::string tune_key(const rrlbm_params& p) {
   char buf[256];
-  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d", p.lx, p.ly, p.lz, p.npop, p.storage,
-           p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged, force_mode_key(p));
+  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d m%d", p.lx, p.ly, p.lz, p.npop,
+           p.storage, p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged,
+           force_mode_key(p), p.collision_model);
   return buf;
 }
 

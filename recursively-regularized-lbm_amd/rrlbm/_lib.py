@@ -31,7 +31,8 @@ class Params(ctypes.Structure):
         ("lid_u", ctypes.c_double * 3), ("f16_scale", ctypes.c_double),
         ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("f16_rounding", ctypes.c_int32),
         ("pop_group_rows", ctypes.c_int32), ("pop_pad_bytes", ctypes.c_int32),
-        ("persistent_wg_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
+        ("persistent_wg_per_cu", ctypes.c_int32), ("collision_model", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -94,6 +95,7 @@ SIGNATURES = {
     "rrlbm_forcing_gp": [_P, _I, ctypes.c_double],
     "rrlbm_les_wale": [_P],
     "rrlbm_collision_rr": [_P],
+    "rrlbm_collision_bgk": [_P],
     "rrlbm_streaming": [_P],
     "rrlbm_bb": [_P],
     "rrlbm_swap_f": [_P],

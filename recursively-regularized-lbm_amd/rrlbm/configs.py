@@ -38,17 +38,23 @@ def cavity(storage=Storage.F32, n=128, npop=27, lid=0.1, re=1000.0, wall_rr=1, *
 
 
 def channel(storage=Storage.F32, lx=384, ly=128, lz=192, re_tau=180.0, u_tau=0.0045, les=False, seed=2024,
-            amp_streak=0.1, amp_noise=0.01, **kw):
+            amp_streak=0.1, amp_noise=0.01, wall_rr=1, **kw):
     h = ly / 2.0
     nu = u_tau * h / re_tau
     s = Solver(lx=lx, ly=ly, lz=lz, npop=27, storage=int(storage), tau=3 * nu + 0.5, force=(u_tau * u_tau / h, 0, 0),
-               wall_y=1, wall_rr=1, use_les=int(les), cw=0.5, **kw)
+               wall_y=1, wall_rr=wall_rr, use_les=int(les), cw=0.5, **kw)
     s.init_channel(u_tau, nu, seed, amp_streak, amp_noise)
     return s, {"nu": nu, "u_tau": u_tau, "h": h, "ystar": nu / u_tau}
 
 
 def channel_les(storage=Storage.F16, **kw):
     return channel(storage=storage, lx=192, ly=64, lz=96, les=True, **kw)
+
+
+def channel_bgk(storage=Storage.F32, **kw):
+    """The channel DNS grid under the BGK baseline operator (BGK/collision.cu:
+    second-order equilibrium, plain bounce-back walls)."""
+    return channel(storage=storage, wall_rr=0, collision_model=1, **kw)
 
 
 def ensemble_instances(seed=5, count=64):

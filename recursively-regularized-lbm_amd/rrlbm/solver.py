@@ -162,6 +162,10 @@ class Solver:
         """LES_WALE, LES.cu:4-98."""
         self._call("rrlbm_les_wale")
 
+    def collision_BGK(self):
+        """collision_BGK, BGK/collision.cu:3-33 (collision_model = 1)."""
+        self._call("rrlbm_collision_bgk")
+
     def collision_RR(self):
         """collision_RR, collision.cu:4-119."""
         self._call("rrlbm_collision_rr")

@@ -368,3 +368,75 @@ def test_laminar_matches_reference_fields():
         assert np.abs(runs[True][k] - ref[None, None, :, col]).max() < 1e-13 * scale
         diff = np.abs(runs[False][k] - ref[None, None, :, col]).max()
         assert 1e-12 < diff < 1e-8  # measured 9.4e-12 (5000 steps), 3.0e-11 (10000)
+
+
+def _bgk_state(seed, lx=6, ly=5, lz=4, force=(2e-4, -1e-4, 5e-5)):
+    rng = np.random.default_rng(seed)
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    _, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    g.f[:] = w[:, None] * 0.02 * rng.standard_normal((27, g.n))   # shifted populations
+    g.set_force(force)
+    g.macrovar_bgk()
+    return g
+
+
+def test_bgk_collision_conserves_mass_and_adds_the_force():
+    """collision_BGK (BGK/collision.cu:3-33) with rho, u from the BGK
+    macrovar (u = j + F/2, BGK/macrovar.cu:27): mass exactly conserved, the
+    momentum of every cell changes by exactly F (relaxation gives F/(2 tau),
+    the forcing term (1 - 1/(2 tau)) F on D3Q27)."""
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    for tau in (0.52, 0.8, 1.7):
+        g = _bgk_state(3)
+        f0 = g.f.copy()
+        g.collision_bgk(tau)
+        assert np.abs(g.f.sum(0) - f0.sum(0)).max() < 1e-15
+        dj = c.T.astype(float) @ (g.f - f0)
+        F = np.array([g.fx, g.fy, g.fz])
+        assert np.abs(dj - F).max() < 1e-16
+
+
+def test_bgk_fixed_points_and_relaxation_rate():
+    """Without force: the rest state and any equilibrium state are fixed
+    points; the non-equilibrium second moment relaxes by exactly 1 - 1/tau."""
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    g = oracle.Fields(oracle.D3Q27_CUDA, 4, 4, 4)
+    g.set_force((0, 0, 0))
+    g.f[:] = 0.0      # shifted populations of the rest state (BGK/initial.cu:131,188-191)
+    g.macrovar_bgk()
+    g.collision_bgk(0.7)
+    assert np.abs(g.f).max() == 0.0
+    # an equilibrium state of the BGK model: feq(rho, u) with u = sum c f
+    rng = np.random.default_rng(8)
+    rho, u = 0.01 * rng.standard_normal(g.n), 0.05 * rng.standard_normal((3, g.n))
+    eu = 3.0 * (c.astype(float) @ u)
+    uv = 3.0 * (u * u).sum(0)
+    g.f[:] = w[:, None] * (rho[None, :] + eu + 0.5 * (eu * eu - uv[None, :]))
+    g.macrovar_bgk()
+    f0 = g.f.copy()
+    g.collision_bgk(0.7)
+    assert np.abs(g.f - f0).max() < 1e-16
+    # relaxation of the second moment
+    g = _bgk_state(5, force=(0, 0, 0))
+    tau = 0.63
+    f0 = g.f.copy()
+    eu = 3.0 * (c.astype(float) @ np.array([g.ux, g.uy, g.uz]))
+    uv = 3.0 * (g.ux ** 2 + g.uy ** 2 + g.uz ** 2)
+    feq = w[:, None] * (g.rho[None, :] + eu + 0.5 * (eu * eu - uv[None, :]))
+    g.collision_bgk(tau)
+    for a in range(3):
+        for b in range(3):
+            m = (c[:, a] * c[:, b]).astype(float)
+            before = m @ (f0 - feq)
+            after = m @ (g.f - feq)
+            assert np.abs(after - (1 - 1 / tau) * before).max() < 1e-16
+
+
+def test_bgk_channel_step_conserves_mass():
+    """The BGK solver's step (collision, periodic streaming, half-way
+    bounce-back at the y walls, macrovar) keeps the total mass of a closed
+    channel to rounding."""
+    g = _bgk_state(11, lx=8, ly=6, lz=5, force=(1e-4, 0, 0))
+    m0 = g.f.sum()
+    g.step_bgk(20, 0.56)
+    assert abs(g.f.sum() - m0) < 1e-14
