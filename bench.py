@@ -275,6 +275,15 @@ def main_configs(args):
         out.append(line_for("MLUPS, channel BGK Re_tau=180 384x128x192 D3Q27", "C3 grid BGK %s" % STORAGE_NAME[st],
                             mlups, ms, 2 * 27 * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup,
                             {"tuning": shape_of(tun)}))
+    # C3 grid with the fourth- to sixth-order RR terms
+    for st in (1, 2):
+        s, meta = configs.channel(storage=st, rr_order=6)
+        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        s.close()
+        log("C3 grid RR order 6 storage %d: %.1f MLUPS" % (st, mlups))
+        out.append(line_for("MLUPS, channel RR (orders 4-6 on) Re_tau=180 384x128x192 D3Q27",
+                            "C3 grid RR6 %s" % STORAGE_NAME[st], mlups, ms, 2 * 27 * STORAGE_BYTES[st],
+                            STORAGE_NAME[st], args.steps, args.warmup, {"tuning": shape_of(tun)}))
     # C2: cavity 128^3 Re 1000, D3Q19 and D3Q27
     for q in (19, 27):
         for st in (1, 0):

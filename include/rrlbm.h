@@ -121,7 +121,10 @@ typedef struct rrlbm_params {
                                    k > 0 = a persistent grid of k workgroups per compute unit
                                    striding over the blocks */
     int32_t collision_model;    /* rrlbm_collision (default RR) */
-    int32_t reserved[1];
+    int32_t rr_order;           /* order of the RR expansion: 3 (0 = default) as the reference
+                                   runs it, or 6 with the fourth- to sixth-order Hermite terms the
+                                   reference keeps commented out (collision.cu:48-49,62-67,95-104,
+                                   112-116; BC.cu:37-38,50-54,79-88,97-101); 6: D3Q27, RR, no LES */
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;

@@ -62,6 +62,39 @@ def hermite(kind):
     return out.reshape(q, 16)
 
 
+def hermite_high(kind):
+    """4th-6th order tables H220 H202 H022 H211 H121 H112 H221 H212 H122 H222."""
+    q = q_of(kind)
+    out = np.zeros(10 * q)
+    load().orc_hermite_high(kind, _p(out))
+    return out.reshape(q, 10)
+
+
+def set_rr_order(order):
+    """3: the reference's RR operator; 6: with its (commented) 4th-6th order terms."""
+    load().orc_set_rr_order(_I(order))
+
+
+def get_rr_order():
+    return load().orc_get_rr_order()
+
+
+class rr_order:
+    """Context manager: the oracle's RR routines at the given order inside."""
+
+    def __init__(self, order):
+        self.order = order
+
+    def __enter__(self):
+        self.prev = get_rr_order()
+        set_rr_order(self.order)
+        return self
+
+    def __exit__(self, *exc):
+        set_rr_order(self.prev)
+        return False
+
+
 def feq(kind, rho, u, v, w):
     out = np.zeros(q_of(kind))
     load().orc_feq(_I(kind), ctypes.c_double(rho), ctypes.c_double(u), ctypes.c_double(v), ctypes.c_double(w),

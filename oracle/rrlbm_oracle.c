@@ -33,6 +33,12 @@
  *                                Taylor_Green/RRBMa0.01L100.txt (20 000 steps)
  *   orc_step_laminar          -> committed FP64 fields Laminar/output/uy*.dat
  *                                (5 000 and 10 000 steps, 1e-14 relative)
+ *   sixth-order RR terms       -> tables: orthogonality of all 27 tensor
+ *   (orc_set_rr_order(6))        Hermite polynomials; feq = w rho prod_a
+ *                                (1 + 3 u_a c_a + 9/2 u_a^2 c_a^2 - 3/2 u_a^2);
+ *                                the non-equilibrium part against its
+ *                                generating-function form (second derivatives
+ *                                of that product), mass and momentum balance
  *   orc_collision_bgk,        -> mass conservation, momentum change = F,
  *   orc_macrovar_bgk             the rest state as a fixed point, the
  *                                second-moment relaxation rate 1 - 1/tau
@@ -62,6 +68,9 @@ typedef struct {
     double H100[27], H010[27], H001[27];
     double H110[27], H101[27], H011[27], H200[27], H020[27], H002[27];
     double H210[27], H201[27], H021[27], H120[27], H102[27], H012[27], H111[27];
+    /* fourth to sixth order (the commented tables of initial.cu:111-130) */
+    double H220[27], H202[27], H022[27], H211[27], H121[27], H112[27];
+    double H221[27], H212[27], H122[27], H222[27];
 } orc_lat;
 
 static void lat_build(int kind, orc_lat* L) {
@@ -96,6 +105,22 @@ static void lat_build(int kind, orc_lat* L) {
         L->H102[i] = x * z * z / cs3 - x / cs;
         L->H012[i] = y * z * z / cs3 - y / cs;
         L->H111[i] = x * y * z / cs3;
+        /* initial.cu:111-130 (commented out there): products of the 1D
+         * polynomials 1, c/cs, c^2/cs^2 - 1.  The reference's H222 line has
+         * + c_y^2 c_z^2 / cs^4 where the product (and orthogonality to H000,
+         * hence mass conservation) needs -; this restatement uses -. */
+        double X2 = x * x / RT, Y2 = y * y / RT, Z2 = z * z / RT;
+        double X1 = x / cs, Y1 = y / cs, Z1 = z / cs;
+        L->H220[i] = X2 * Y2 - X2 - Y2 + 1;
+        L->H202[i] = X2 * Z2 - X2 - Z2 + 1;
+        L->H022[i] = Y2 * Z2 - Y2 - Z2 + 1;
+        L->H211[i] = X2 * Y1 * Z1 - Y1 * Z1;
+        L->H121[i] = X1 * Y2 * Z1 - X1 * Z1;
+        L->H112[i] = X1 * Y1 * Z2 - X1 * Y1;
+        L->H221[i] = X2 * Y2 * Z1 - X2 * Z1 - Y2 * Z1 + Z1;
+        L->H212[i] = X2 * Y1 * Z2 - X2 * Y1 - Y1 * Z2 + Y1;
+        L->H122[i] = X1 * Y2 * Z2 - X1 * Y2 - X1 * Z2 + X1;
+        L->H222[i] = X2 * Y2 * Z2 - X2 * Y2 - X2 * Z2 - Y2 * Z2 + X2 + Y2 + Z2 - 1;
     }
     for (int i = 0; i < L->q; ++i)
         for (int j = 0; j < L->q; ++j)
@@ -125,10 +150,35 @@ void orc_hermite(int kind, double* out /* 16*q */) {
     }
 }
 
+/* Tables 4th-6th order for direction i, in the order
+ * H220 H202 H022 H211 H121 H112 H221 H212 H122 H222 */
+void orc_hermite_high(int kind, double* out /* 10*q */) {
+    orc_lat L;
+    lat_build(kind, &L);
+    for (int i = 0; i < L.q; ++i) {
+        double* o = out + 10 * i;
+        o[0] = L.H220[i]; o[1] = L.H202[i]; o[2] = L.H022[i];
+        o[3] = L.H211[i]; o[4] = L.H121[i]; o[5] = L.H112[i];
+        o[6] = L.H221[i]; o[7] = L.H212[i]; o[8] = L.H122[i]; o[9] = L.H222[i];
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Equilibrium and the RR operator for one cell                        */
 /* ------------------------------------------------------------------ */
-typedef struct { double a110, a101, a011, a200, a020, a002, a210, a201, a120, a021, a012, a102, a111; } orc_a;
+typedef struct {
+    double a110, a101, a011, a200, a020, a002, a210, a201, a120, a021, a012, a102, a111;
+    double a220, a202, a022, a211, a121, a112, a221, a212, a122, a222;
+} orc_a;
+
+/* Order of the RR expansion: 3 (the reference's active code) or 6 (the
+ * fourth- to sixth-order terms the reference carries commented out,
+ * collision.cu:48-49,62-67,95-104,112-116 and BC.cu:37-38,50-54,79-88,97-101;
+ * D3Q27 only). */
+static int g_rr_order = 3;
+void orc_set_rr_order(int order) { g_rr_order = order == 6 ? 6 : 3; }
+int orc_get_rr_order(void) { return g_rr_order; }
+static int high_order(const orc_lat* L) { return g_rr_order == 6 && L->q == 27; }
 
 /* Third-order Hermite sum for direction ip.
  * D3Q27: collision.cu:60-61.  D3Q19: Malaspinas pairs, D3Q19RR.f90:131-136,230-235. */
@@ -154,6 +204,23 @@ static void eq_coeffs(double uu, double vv, double ww, orc_a* a) {
     a->a200 = uu * uu; a->a020 = vv * vv; a->a002 = ww * ww;
     a->a210 = a->a200 * vv; a->a201 = a->a200 * ww; a->a120 = a->a020 * uu; a->a021 = a->a020 * ww;
     a->a012 = a->a002 * vv; a->a102 = a->a002 * uu; a->a111 = a->a110 * ww;
+    /* collision.cu:48-49 (commented there) */
+    a->a220 = a->a210 * vv; a->a202 = a->a201 * ww; a->a022 = a->a021 * ww;
+    a->a211 = a->a210 * ww; a->a121 = a->a120 * ww; a->a112 = a->a102 * vv;
+    a->a221 = a->a220 * ww; a->a212 = a->a202 * vv; a->a122 = a->a022 * uu; a->a222 = a->a221 * ww;
+}
+
+/* four = H220 a220 + H202 a202 + H022 a022 + 2 (H211 a211 + H121 a121 + H112 a112),
+ * enters as four/4 (collision.cu:62-63,66) */
+static double fourth_sum(const orc_lat* L, int ip, const orc_a* a) {
+    return L->H220[ip] * a->a220 + L->H202[ip] * a->a202 + L->H022[ip] * a->a022 +
+           2.0 * (L->H211[ip] * a->a211 + L->H121[ip] * a->a121 + L->H112[ip] * a->a112);
+}
+
+/* fifsix = (H221 a221 + H212 a212 + H122 a122)/4 + H222 a222/8 (collision.cu:64) */
+static double fifsix_sum(const orc_lat* L, int ip, const orc_a* a) {
+    return (L->H221[ip] * a->a221 + L->H212[ip] * a->a212 + L->H122[ip] * a->a122) / 4.0 +
+           L->H222[ip] * a->a222 / 8.0;
 }
 
 /* feq_i = w_i rho (1 + H1.u~ + (sec + thir)/2)   (collision.cu:57-69) */
@@ -163,7 +230,8 @@ static void feq_cell(const orc_lat* L, double rho, double u, double v, double w,
     eq_coeffs(uu, vv, ww, &a);
     for (int ip = 0; ip < L->q; ++ip) {
         double fir = L->H100[ip] * uu + L->H010[ip] * vv + L->H001[ip] * ww;
-        feq[ip] = L->w[ip] * rho * (1.0 + fir + 0.5 * (second_sum(L, ip, &a) + third_sum(L, ip, &a)));
+        double hi = high_order(L) ? fourth_sum(L, ip, &a) / 4.0 + fifsix_sum(L, ip, &a) : 0.0;
+        feq[ip] = L->w[ip] * rho * (1.0 + fir + 0.5 * (second_sum(L, ip, &a) + third_sum(L, ip, &a)) + hi);
     }
 }
 
@@ -190,15 +258,22 @@ static void fbar_cell(const orc_lat* L, double u, double v, double w, double fx,
  * regularisation passes a Fbar computed at another cell). */
 static void rr_cell_fbar(const orc_lat* L, const double* f, double rho, double u, double v, double w,
                          const double* Fbar, double tau_eff, int mode, double* out) {
-    double feq[27];
+    /* feq: the equilibrium; ffeq: the equilibrium the force term multiplies
+     * (forcefeq, collision.cu:66-67: the same at third order, without the
+     * fifth/sixth-order terms at sixth order) */
+    double feq[27], ffeq[27];
+    const int ho = high_order(L);
     double uu = u / ORC_CS, vv = v / ORC_CS, ww = w / ORC_CS;
     orc_a a;
     eq_coeffs(uu, vv, ww, &a);
     double ane110 = 0, ane101 = 0, ane011 = 0, ane200 = 0, ane020 = 0, ane002 = 0;
     for (int ip = 0; ip < L->q; ++ip) {
         double fir = L->H100[ip] * uu + L->H010[ip] * vv + L->H001[ip] * ww;
-        feq[ip] = L->w[ip] * rho * (1.0 + fir + 0.5 * (second_sum(L, ip, &a) + third_sum(L, ip, &a)));
-        double g = f[ip] + Fbar[ip] * feq[ip];
+        double base = 1.0 + fir + 0.5 * (second_sum(L, ip, &a) + third_sum(L, ip, &a));
+        if (ho) base += fourth_sum(L, ip, &a) / 4.0;
+        ffeq[ip] = L->w[ip] * rho * base;
+        feq[ip] = ho ? L->w[ip] * rho * (base + fifsix_sum(L, ip, &a)) : ffeq[ip];
+        double g = f[ip] + Fbar[ip] * ffeq[ip];
         ane110 += L->H110[ip] * g; ane101 += L->H101[ip] * g; ane011 += L->H011[ip] * g;
         ane200 += L->H200[ip] * g; ane020 += L->H020[ip] * g; ane002 += L->H002[ip] * g;
     }
@@ -213,12 +288,33 @@ static void rr_cell_fbar(const orc_lat* L, const double* f, double rho, double u
     n.a102 = n.a002 * uu + 2. * ww * n.a101;
     n.a012 = n.a002 * vv + 2. * ww * n.a011;
     n.a111 = n.a110 * ww + n.a101 * vv + n.a011 * uu;
+    if (ho) {
+        /* collision.cu:95-104 (commented there).  Fourth order as written; in
+         * the fifth/sixth-order lines the reference multiplies by a_200 =
+         * rho uu^2 (and a_020, a_110, a_011, a_101) where the recursion of its
+         * own lower orders (u products only, as in Malaspinas 2015) has
+         * uu^2: this restatement uses the u products, so every order follows
+         * one recursion. */
+        n.a220 = n.a210 * vv + uu * uu * n.a020 + 2. * uu * vv * n.a110;
+        n.a202 = n.a201 * ww + uu * uu * n.a002 + 2. * uu * ww * n.a101;
+        n.a022 = n.a021 * ww + vv * vv * n.a002 + 2. * vv * ww * n.a011;
+        n.a211 = n.a210 * ww + uu * uu * n.a011 + 2. * uu * vv * n.a101;
+        n.a121 = n.a120 * ww + 2. * uu * vv * n.a011 + vv * vv * n.a101;
+        n.a112 = n.a111 * ww + uu * vv * n.a002 + uu * ww * n.a011 + vv * ww * n.a101;
+        n.a221 = n.a220 * ww + 2. * uu * uu * vv * n.a011 + 2. * vv * vv * uu * n.a101;
+        n.a212 = n.a211 * ww + 2. * uu * vv * ww * n.a101 + uu * uu * (vv * n.a002 + ww * n.a011);
+        n.a122 = n.a121 * ww + 2. * uu * vv * ww * n.a011 + vv * vv * (uu * n.a002 + ww * n.a101);
+        n.a222 = n.a221 * ww + uu * uu * vv * vv * n.a002 + 2. * uu * uu * vv * ww * n.a011 +
+                 2. * uu * vv * vv * ww * n.a101;
+    }
     for (int ip = 0; ip < L->q; ++ip) {
-        double fneq = L->w[ip] * (0.5 * (second_sum(L, ip, &n) + third_sum(L, ip, &n)));
+        double fneq = 0.5 * (second_sum(L, ip, &n) + third_sum(L, ip, &n));
+        if (ho) fneq += fourth_sum(L, ip, &n) / 4.0 + fifsix_sum(L, ip, &n); /* collision.cu:112-116 */
+        fneq *= L->w[ip];
         if (mode == 0)
-            out[ip] = feq[ip] + (1.0 - 1.0 / tau_eff) * fneq + Fbar[ip] * feq[ip];
+            out[ip] = feq[ip] + (1.0 - 1.0 / tau_eff) * fneq + Fbar[ip] * ffeq[ip];
         else
-            out[ip] = feq[ip] + fneq - Fbar[ip] * feq[ip];
+            out[ip] = feq[ip] + fneq - Fbar[ip] * ffeq[ip];
     }
 }
 

@@ -34,6 +34,9 @@ RRLBM_HD RRLBM_INLINE uint32_t fastdiv(uint32_t n, const FastDiv& f) {
 
 enum Storage : int { kF64 = 0, kF32 = 1, kF16 = 2 };
 enum Mode : int { kStep = 0, kMacro = 1, kFinish = 2, kDiag = 3 };
+// collision operator of a fused kernel: third-order RR, BGK baseline, RR with
+// the fourth- to sixth-order terms
+enum FusedOp : int { kOpRR = 0, kOpBGK = 1, kOpRR6 = 2 };
 
 template <int S>
 struct StoreT;
@@ -131,8 +134,10 @@ struct FusedArgs {
 // Host launchers (kernels_fused.hip)
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s);
-// The BGK baseline operator (D3Q27, no LES): step, finish and diagnostics modes.
-hipError_t launch_fused_bgk(int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s);
+// The alternative operators op = kOpBGK / kOpRR6 (D3Q27, no LES): step, finish
+// and diagnostics modes.
+hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid,
+                           hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list

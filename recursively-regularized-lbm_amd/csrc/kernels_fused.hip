@@ -124,7 +124,7 @@ RRLBM_HD constexpr bool enters(int face, int x, int y, int z) {
   }
 }
 
-template <class L, typename C, int face, int FM>
+template <class L, typename C, int face, int FM, bool O6>
 RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
   Moments<C> m;
   if constexpr (L::Q == 27) m = moments_q27<C>(h);
@@ -133,7 +133,10 @@ RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
   C out[L::Q];
   if constexpr (L::Q == 27) {
     const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
-    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
+    if constexpr (O6)
+      rr_post_q27_o6<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
+    else
+      rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
   } else {
     rr_post_generic<L, C>(h, q, Fx, Fy, Fz, C(1), C(-1), out);
   }
@@ -157,8 +160,9 @@ RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, const
   return v;
 }
 
-// off: this cell's byte offset from `base` in the source buffer (without i*seg)
-template <int S, class L, int FM>
+// off: this cell's byte offset from `base` in the source buffer (without i*seg);
+// O6: regularise with the fourth- to sixth-order terms
+template <int S, class L, int FM, bool O6 = false>
 RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
                                             uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off) {
   using C = typename StoreT<S>::C;
@@ -189,7 +193,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
         if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
       });
     }
-    regularise_face<L, C, face, FM>(h, Fx, Fy, Fz);
+    regularise_face<L, C, face, FM, O6>(h, Fx, Fy, Fz);
     done = true;
   });
 }
@@ -260,9 +264,11 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
 // then wavefront-uniform (scalar registers): each population's source address
 // is a scalar row base plus one of three per-lane x offsets, instead of a
 // per-lane three-term sum.  Otherwise (any lx) every lane computes its own.
-// BGK: the baseline BGK operator (bgk_post) instead of the RR collision, with
-// the BGK solver's velocity u = j + F/2 (D3Q27, no LES, no wall regularisation).
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, bool BGK = false>
+// OP: kOpRR the RR collision; kOpBGK the baseline BGK operator (bgk_post) with
+// the BGK solver's velocity u = j + F/2 (D3Q27, no LES, no wall
+// regularisation); kOpRR6 the RR collision and wall regularisation with the
+// fourth- to sixth-order terms (rr_post_q27_o6, D3Q27).
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, int OP = kOpRR>
 RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
   using T = typename StoreT<S>::T;
@@ -329,7 +335,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-    if (on_wall) wall_treatment<S, L, FM>(a, h, x, y, z, own_src, own_off);
+    if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off);
   }
 
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
@@ -337,7 +343,7 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   if constexpr (Q == 27) m = moments_q27<C>(h);
   else m = moments_generic<L, C>(h);
   Macro<C> q;
-  if constexpr (BGK) {
+  if constexpr (OP == kOpBGK) {
     q.dr = m.dr;
     q.rho = C(1) + m.dr;
     q.ux = m.jx + C(0.5) * Fx;
@@ -432,13 +438,20 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     }
     if constexpr (!LES) omega = C(a.omega);
     C out[Q];
-    if constexpr (BGK) {
+    if constexpr (OP == kOpBGK) {
       const C inv_tau = C(1) - omega;
       const C fcoef = C(1) - C(0.5) * inv_tau;
       if constexpr (S == kF16)
         bgk_post<L, C, false>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(a.f16_scale), out);
       else
         bgk_post<L, C, true>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(1), out);
+    } else if constexpr (OP == kOpRR6) {
+      static_assert(Q == 27, "the sixth-order terms need D3Q27");
+      const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+      if constexpr (S == kF16)
+        rr_post_q27_o6<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
+      else
+        rr_post_q27_o6<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
     } else if constexpr (Q == 27) {
       const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
       if constexpr (S == kF16)
@@ -473,21 +486,21 @@ RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
 }
 
 // One workgroup per block of 256 cells (grid = a.tiles).
-template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, bool BGK = false>
+template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, int OP = kOpRR>
 __global__ __launch_bounds__(kBlock) void fused_kernel(FusedArgs a) {
   const uint32_t b = block_of(a, blockIdx.x);
-  if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS, BGK>(a, b);
+  if (b < a.nblocks) fused_body<S, L, WALLS, LES, MODE, FM, ROWS, OP>(a, b);
 }
 
 // Persistent grid (step mode): fewer workgroups than blocks, each striding
 // over the block sequence; every block is computed once per launch.  A
 // separate kernel, because the loop raises the register count of the body
 // (loop-invariant addresses stay live), which one-block workgroups avoid.
-template <int S, class L, bool WALLS, bool LES, int FM, bool ROWS, bool BGK = false>
+template <int S, class L, bool WALLS, bool LES, int FM, bool ROWS, int OP = kOpRR>
 __global__ __launch_bounds__(kBlock) void fused_kernel_persistent(FusedArgs a) {
   for (uint32_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
     const uint32_t b = block_of(a, t);
-    if (b < a.nblocks) fused_body<S, L, WALLS, LES, kStep, FM, ROWS, BGK>(a, b);
+    if (b < a.nblocks) fused_body<S, L, WALLS, LES, kStep, FM, ROWS, OP>(a, b);
   }
 }
 
@@ -719,39 +732,49 @@ hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32
   }
 }
 
-// The BGK baseline: D3Q27, no LES, general force (step, finish and
-// diagnostics passes).
-template <int S, bool W, bool R>
-static hipError_t launch_bgk_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+// The alternative operators (BGK baseline, sixth-order RR): D3Q27, no LES,
+// general force (step, finish and diagnostics passes).
+template <int S, bool W, bool R, int OP>
+static hipError_t launch_op_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   if (mode == kStep && grid < a.tiles) {
-    hipLaunchKernelGGL((fused_kernel_persistent<S, D3Q27, W, false, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0,
+    hipLaunchKernelGGL((fused_kernel_persistent<S, D3Q27, W, false, kForceAll, R, OP>), dim3(grid), dim3(kBlock), 0,
                        s, a);
     return hipGetLastError();
   }
   switch (mode) {
-    case kStep: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kStep, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kFinish: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kFinish, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case kDiag: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kDiag, kForceAll, R, true>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kStep: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kStep, kForceAll, R, OP>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kFinish: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kFinish, kForceAll, R, OP>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case kDiag: hipLaunchKernelGGL((fused_kernel<S, D3Q27, W, false, kDiag, kForceAll, R, OP>), dim3(grid), dim3(kBlock), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int S>
-static hipError_t launch_bgk_storage(bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+template <int S, int OP>
+static hipError_t launch_op_storage(bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   const bool rows = a.lx % 64u == 0;
   if (walls)
-    return rows ? launch_bgk_mode<S, true, true>(mode, a, grid, s) : launch_bgk_mode<S, true, false>(mode, a, grid, s);
-  return rows ? launch_bgk_mode<S, false, true>(mode, a, grid, s) : launch_bgk_mode<S, false, false>(mode, a, grid, s);
+    return rows ? launch_op_mode<S, true, true, OP>(mode, a, grid, s)
+                : launch_op_mode<S, true, false, OP>(mode, a, grid, s);
+  return rows ? launch_op_mode<S, false, true, OP>(mode, a, grid, s)
+              : launch_op_mode<S, false, false, OP>(mode, a, grid, s);
 }
 
-hipError_t launch_fused_bgk(int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+template <int OP>
+static hipError_t launch_op(int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   switch (storage) {
-    case kF64: return launch_bgk_storage<kF64>(walls, mode, a, grid, s);
-    case kF32: return launch_bgk_storage<kF32>(walls, mode, a, grid, s);
-    case kF16: return launch_bgk_storage<kF16>(walls, mode, a, grid, s);
+    case kF64: return launch_op_storage<kF64, OP>(walls, mode, a, grid, s);
+    case kF32: return launch_op_storage<kF32, OP>(walls, mode, a, grid, s);
+    case kF16: return launch_op_storage<kF16, OP>(walls, mode, a, grid, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid,
+                           hipStream_t s) {
+  if (op == kOpBGK) return launch_op<kOpBGK>(storage, walls, mode, a, grid, s);
+  if (op == kOpRR6) return launch_op<kOpRR6>(storage, walls, mode, a, grid, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,

@@ -60,7 +60,10 @@ __global__ __launch_bounds__(kBlock) void k_collision(RefFields r, Grid g, doubl
   if constexpr (Q == 27) {
     const Moments<double> m = moments_q27<double>(h);
     const NeqTensor<double> N = neq_q27<double>(m, q, Fx, Fy, Fz);
-    rr_post_q27<double, true>(q, N, Fx, Fy, Fz, omega, 1.0, 1.0, out);
+    if (g.rr6)
+      rr_post_q27_o6<double, true>(q, N, Fx, Fy, Fz, omega, 1.0, 1.0, out);
+    else
+      rr_post_q27<double, true>(q, N, Fx, Fy, Fz, omega, 1.0, 1.0, out);
   } else {
     rr_post_generic<L, double>(h, q, Fx, Fy, Fz, omega, 1.0, out);
   }
@@ -200,7 +203,10 @@ __global__ __launch_bounds__(kBlock) void k_rrbc(RefFields r, const double* fpos
     if constexpr (Q == 27) {
       const Moments<double> m = moments_q27<double>(h);
       const NeqTensor<double> N = neq_q27<double>(m, q, Fx, Fy, Fz);
-      rr_post_q27<double, true>(q, N, Fx, Fy, Fz, 1.0, -1.0, 1.0, out);
+      if (g.rr6)
+        rr_post_q27_o6<double, true>(q, N, Fx, Fy, Fz, 1.0, -1.0, 1.0, out);
+      else
+        rr_post_q27<double, true>(q, N, Fx, Fy, Fz, 1.0, -1.0, 1.0, out);
     } else {
       rr_post_generic<L, double>(h, q, Fx, Fy, Fz, 1.0, -1.0, out);
     }
@@ -327,6 +333,18 @@ __global__ __launch_bounds__(kBlock) void k_init_equilibrium(RefFields r, Grid g
       const double cu = L::cx(i) * ux + L::cy(i) * uy + L::cz(i) * uz;
       out[i] = L::w(i) * (r.rho[cell] - 1.0 + 3.0 * cu + 4.5 * cu * cu - uu);
     });
+  } else if constexpr (L::Q == 27) {
+    if (g.rr6) {
+      // the full-order equilibrium: the sixth-order post-collision state of a
+      // cell without non-equilibrium part or force
+      Macro<double> q;
+      q.rho = r.rho[cell]; q.dr = q.rho - 1.0;
+      q.ux = r.ux[cell]; q.uy = r.uy[cell]; q.uz = r.uz[cell];
+      const NeqTensor<double> N{0, 0, 0, 0, 0, 0};
+      rr_post_q27_o6<double, true, kForceNone>(q, N, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0, out);
+    } else {
+      feq_shifted<L, double>(r.rho[cell], r.ux[cell], r.uy[cell], r.uz[cell], out);
+    }
   } else {
     feq_shifted<L, double>(r.rho[cell], r.ux[cell], r.uy[cell], r.uz[cell], out);
   }
@@ -417,7 +435,10 @@ __global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst,
     } else {
       const Moments<double> m = moments_q27<double>(h);
       const NeqTensor<double> N = neq_q27<double>(m, q, fx, fy, fz);
-      rr_post_q27<double, true>(q, N, fx, fy, fz, omega, 1.0, 1.0, out);
+      if (g.rr6)
+        rr_post_q27_o6<double, true>(q, N, fx, fy, fz, omega, 1.0, 1.0, out);
+      else
+        rr_post_q27<double, true>(q, N, fx, fy, fz, omega, 1.0, 1.0, out);
     }
   } else {
     rr_post_generic<L, double>(h, q, fx, fy, fz, omega, 1.0, out);

@@ -13,6 +13,7 @@ struct Grid {
   uint32_t walls;  // bit 0 x faces, bit 1 y faces, bit 2 z faces
   int32_t lid_face;
   double lid_x, lid_y, lid_z;
+  int32_t rr6;     // RR operators with the fourth- to sixth-order terms (D3Q27)
 };
 
 // The channel solver's device fields (RR27V33_code_DNS_LES/globals.cu:67-71).

@@ -348,6 +348,81 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
   });
 }
 
+// The same state with every Hermite order D3Q27 carries: the fourth- to
+// sixth-order terms of the RR expansion, which the reference keeps commented
+// out (collision.cu:48-49,62-67,95-104,112-116; BC.cu:37-38,50-54,79-88,97-101).
+// Outputs as rr_post_q27.  With these orders the equilibrium is a product over
+// the axes, G = prod_a P_a(c_a), P(c) = 1 + 3uc + 9/2 u^2 c^2 - 3/2 u^2, and the
+// recursive non-equilibrium part is (1/2) sum_ab A_ab d2G/du~_a du~_b:
+//   feq / w  = rho G
+//   fneq / w = 3/2 sum_a N_aa h_a prod_{b!=a} P_b + 9 sum_{a<b} N_ab D_a D_b P_c,
+//              h = 3c^2 - 1, D = c + h u
+//   force:     Fbar times the equilibrium truncated after fourth order
+//              (forcefeq of collision.cu:67), Q = G - T2 T2 T1 (3 ways) - T2 T2 T2,
+//              T1 = 3uc, T2 = 3/2 u^2 h (the per-axis first / second-order parts of P).
+// Everything is evaluated at c in {-1, 0, 1} per axis, then combined over
+// (x, y) pairs and finally z; the shift by w is taken analytically
+// (out / w - 1 = dr + rho (G - 1) + ...) with G - 1 formed from the small
+// per-axis parts p = P - 1, so FP32 keeps the relative precision of h.
+template <typename C, bool per_direction_w, int FM = kForceAll>
+RRLBM_HD RRLBM_INLINE void rr_post_q27_o6(const Macro<C>& q, const NeqTensor<C>& N, C Fx, C Fy, C Fz, C omega,
+                                          C gs, C scale, C (&out)[27]) {
+  struct Axis {
+    C p[3], P[3], T1[3], T2[3], D[3], g[3];
+  };
+  auto axis = [](C u, C F) {
+    Axis a;
+    const C u2 = u * u, t = C(3) * u, s = C(3) * u2;
+    a.p[0] = s - t; a.p[1] = C(-1.5) * u2; a.p[2] = s + t;
+    a.T1[0] = -t; a.T1[1] = C(0); a.T1[2] = t;
+    a.T2[0] = s; a.T2[1] = C(-1.5) * u2; a.T2[2] = s;
+    a.D[0] = C(2) * u - C(1); a.D[1] = -u; a.D[2] = C(2) * u + C(1);
+    const C f15 = C(1.5) * F;
+    a.g[0] = -f15 * (C(1) + u); a.g[1] = -f15 * u; a.g[2] = f15 * (C(1) - u);
+    for (int j = 0; j < 3; ++j) a.P[j] = C(1) + a.p[j];
+    return a;
+  };
+  constexpr bool fx = FM != kForceNone, fyz = FM == kForceAll;
+  const Axis X = axis(q.ux, fx ? Fx : C(0)), Y = axis(q.uy, fyz ? Fy : C(0)), Z = axis(q.uz, fyz ? Fz : C(0));
+  // h(0) = -1, h(+-1) = 2
+  auto hval = [](int j) { return j == 1 ? C(-1) : C(2); };
+  const C nxx = C(1.5) * omega * N.xx, nyy = C(1.5) * omega * N.yy, nzz = C(1.5) * omega * N.zz;
+  const C nxy = C(9) * omega * N.xy, nxz = C(9) * omega * N.xz, nyz = C(9) * omega * N.yz;
+  const C grho = gs * q.rho;
+  static_for<3>([&](auto jxc) {
+    constexpr int jx = decltype(jxc)::value;
+    const C ax = nxx * hval(jx), dx = nxy * X.D[jx], ex = nxz * X.D[jx];
+    static_for<3>([&](auto jyc) {
+      constexpr int jy = decltype(jyc)::value;
+      const C Exy = X.p[jx] + Y.p[jy] + X.p[jx] * Y.p[jy];
+      const C PP = C(1) + Exy;
+      const C Ap = ax * Y.P[jy] + (nyy * hval(jy)) * X.P[jx] + dx * Y.D[jy];
+      const C Bp = nzz * PP;
+      const C Cp = ex * Y.P[jy] + (nyz * Y.D[jy]) * X.P[jx];
+      C TT = C(0), TU = C(0), Sxy = C(0);
+      if constexpr (fx) {
+        TT = X.T2[jx] * Y.T2[jy];
+        TU = X.T2[jx] * Y.T1[jy] + X.T1[jx] * Y.T2[jy];
+        Sxy = X.g[jx] + Y.g[jy];
+      }
+      static_for<3>([&](auto jzc) {
+        constexpr int jz = decltype(jzc)::value;
+        constexpr int i = D3Q27::index(jx - 1, jy - 1, jz - 1);
+        const C E = Exy + Z.p[jz] * PP;
+        C val = q.dr + q.rho * E + Ap * Z.P[jz] + Bp * hval(jz) + Cp * Z.D[jz];
+        if constexpr (fx) {
+          const C Qv = PP * Z.P[jz] - TT * Z.p[jz] - TU * Z.T2[jz];
+          val += grho * (Sxy + Z.g[jz]) * Qv;
+        }
+        if constexpr (per_direction_w)
+          out[i] = val * C(D3Q27::w(i));
+        else
+          out[i] = val * scale;
+      });
+    });
+  });
+}
+
 // Per-direction formulation, following collision.cu:41-118 (D3Q27) and the
 // D3Q19 third-order pairs of D3Q19RR.f90:131-136,221-235,263-281.  h are the
 // shifted populations of the cell; writes out_i = g_i - w_i with

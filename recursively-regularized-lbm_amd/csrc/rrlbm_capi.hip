@@ -126,6 +126,7 @@ double* field_ptr(rrlbm_ctx* c, rrlbm_field f, size_t* count) {
 
 bool has_walls(const rrlbm_params& p) { return p.wall_x || p.wall_y || p.wall_z; }
 bool is_bgk(const rrlbm_params& p) { return p.collision_model == RRLBM_COLLISION_BGK; }
+bool is_rr6(const rrlbm_params& p) { return p.collision_model == RRLBM_COLLISION_RR && p.rr_order == 6; }
 
 void set_blocks(FusedArgs& a, uint32_t nblocks, int remap) {
   a.nblocks = nblocks;
@@ -210,9 +211,10 @@ uint32_t fused_grid(const rrlbm_ctx* c, const FusedArgs& a) {
 
 int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
   const bool les = c->p.use_les && mode == kStep;
-  hipError_t e = is_bgk(c->p) ? launch_fused_bgk(c->p.storage, has_walls(c->p), mode, a, fused_grid(c, a), c->stream)
-                              : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a),
-                                             c->stream);
+  const int op = is_bgk(c->p) ? kOpBGK : (is_rr6(c->p) ? kOpRR6 : kOpRR);
+  hipError_t e = op != kOpRR ? launch_fused_op(op, c->p.storage, has_walls(c->p), mode, a, fused_grid(c, a), c->stream)
+                             : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a),
+                                            c->stream);
   if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
   return RRLBM_OK;
 }
@@ -220,7 +222,7 @@ int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
 // Two cells per thread for the plain D3Q27 step only when requested: on
 // MI355X one cell per thread is faster for every storage (higher occupancy).
 bool use_pair(const rrlbm_ctx* c) {
-  if (c->q != 27 || c->p.use_les || is_bgk(c->p)) return false;
+  if (c->q != 27 || c->p.use_les || is_bgk(c->p) || is_rr6(c->p)) return false;
   return c->p.cells_per_thread == 2;
 }
 
@@ -408,6 +410,10 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     return fail(nullptr, RRLBM_E_INVALID, "collision_model must be 0 (RR) or 1 (BGK)");
   if (p.collision_model == RRLBM_COLLISION_BGK && (p.npop != 27 || p.use_les || p.wall_rr))
     return fail(nullptr, RRLBM_E_UNSUPPORTED, "the BGK baseline is D3Q27 without LES and without RR wall regularisation");
+  if (p.rr_order == 0) p.rr_order = 3;
+  if (p.rr_order != 3 && p.rr_order != 6) return fail(nullptr, RRLBM_E_INVALID, "rr_order must be 0, 3 or 6");
+  if (p.rr_order == 6 && (p.collision_model != RRLBM_COLLISION_RR || p.npop != 27 || p.use_les))
+    return fail(nullptr, RRLBM_E_UNSUPPORTED, "rr_order 6 is the D3Q27 RR operator without LES");
   if (p.cells_per_thread < 0 || p.cells_per_thread > 2)
     return fail(nullptr, RRLBM_E_INVALID, "cells_per_thread must be 0, 1 or 2");
   if (p.persistent_wg_per_cu < 0 || p.persistent_wg_per_cu > 64)
@@ -433,6 +439,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   c->grid.walls = (p.wall_x ? 1u : 0u) | (p.wall_y ? 2u : 0u) | (p.wall_z ? 4u : 0u);
   c->grid.lid_face = p.lid_face;
   c->grid.lid_x = p.lid_u[0]; c->grid.lid_y = p.lid_u[1]; c->grid.lid_z = p.lid_u[2];
+  c->grid.rr6 = is_rr6(p) ? 1 : 0;
   c->nblocks = (c->n + kBlock - 1) / kBlock;
   c->lay = lay;
 
@@ -887,9 +894,9 @@ int force_mode_key(const rrlbm_params& p) {
 
 std::string tune_key(const rrlbm_params& p) {
   char buf[256];
-  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d m%d", p.lx, p.ly, p.lz, p.npop,
+  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d m%d o%d", p.lx, p.ly, p.lz, p.npop,
            p.storage, p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged,
-           force_mode_key(p), p.collision_model);
+           force_mode_key(p), p.collision_model, p.rr_order);
   return buf;
 }
 

@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
         ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("f16_rounding", ctypes.c_int32),
         ("pop_group_rows", ctypes.c_int32), ("pop_pad_bytes", ctypes.c_int32),
         ("persistent_wg_per_cu", ctypes.c_int32), ("collision_model", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("rr_order", ctypes.c_int32),
     ]
 
 

@@ -24,6 +24,21 @@ static void scaled_q27(const C (&h)[27], const Macro<C>& q, const double* F, C o
   for (int i = 0; i < 27; ++i) o[i] = C(double(o[i]) * D3Q27::w(i) / kScale);
 }
 
+// The same with the fourth- to sixth-order terms (rr_post_q27_o6); scaled:
+// the FP16-storage form.
+template <typename C, int FM, bool scaled>
+static void o6_q27(const C (&h)[27], const Macro<C>& q, const double* F, C omega, C gs, C (&o)[27]) {
+  constexpr double kScale = 4096.0;
+  const Moments<C> m = moments_q27<C>(h);
+  const NeqTensor<C> N = neq_q27<C, FM>(m, q, C(F[0]), C(F[1]), C(F[2]));
+  if constexpr (scaled) {
+    rr_post_q27_o6<C, false, FM>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(kScale), o);
+    for (int i = 0; i < 27; ++i) o[i] = C(double(o[i]) * D3Q27::w(i) / kScale);
+  } else {
+    rr_post_q27_o6<C, true, FM>(q, N, C(F[0]), C(F[1]), C(F[2]), omega, gs, C(1), o);
+  }
+}
+
 template <class L, typename C>
 static void cell(const double* f, double rho, double ux, double uy, double uz, const double* F, double tau_eff,
                  int mode, int fast, double* out) {
@@ -46,6 +61,14 @@ static void cell(const double* f, double rho, double ux, double uy, double uz, c
       scaled_q27<C, kForceX>(h, q, F, omega, gs, o);
     } else if (fast == 6) {
       scaled_q27<C, kForceNone>(h, q, F, omega, gs, o);
+    } else if (fast == 7) {
+      o6_q27<C, kForceAll, false>(h, q, F, omega, gs, o);
+    } else if (fast == 8) {
+      o6_q27<C, kForceX, false>(h, q, F, omega, gs, o);
+    } else if (fast == 9) {
+      o6_q27<C, kForceNone, false>(h, q, F, omega, gs, o);
+    } else if (fast == 10) {
+      o6_q27<C, kForceAll, true>(h, q, F, omega, gs, o);
     } else {
       rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
     }
@@ -69,7 +92,8 @@ static void moments(const double* f, double* m) {
 extern "C" {
 // q: 27 or 19; single: 1 = FP32 arithmetic; fast: 0 per-direction path, 1/2/3 moment-space D3Q27
 // path with all force components / force along x only / no force, 4/5/6 the same in the
-// FP16-storage form (values scaled by 4096 / w_i)
+// FP16-storage form (values scaled by 4096 / w_i); 7/8/9 the sixth-order path with all force
+// components / x only / none, 10 its FP16-storage form
 void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double ux, double uy, double uz,
                 const double* F, double tau_eff, int mode, double* out) {
   if (q == 27) {

@@ -132,3 +132,36 @@ def test_wale_kernel_formula(hc):
         got = hc.hc_wale(p(flat), ctypes.c_double(0.5))
         assert abs(got - wale_matrix(g)) <= 1e-13 * abs(wale_matrix(g)) + 1e-30
     assert hc.hc_wale(p(np.zeros(9)), ctypes.c_double(0.5)) == 0.0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("fast,force", [(7, "all"), (8, "x"), (9, "none"), (10, "all")])
+@pytest.mark.parametrize("consistent", [True, False])
+def test_sixth_order_cell_matches_oracle(hc, fast, force, mode, consistent):
+    """The kernels' factorised sixth-order form (rr_post_q27_o6) against the
+    oracle's per-direction Hermite sums with the fourth- to sixth-order terms."""
+    rng = np.random.default_rng(30 + fast + 4 * mode)
+    with oracle.rr_order(6):
+        for _ in range(50):
+            f, rho, u, F = random_cell(oracle.D3Q27_CUDA, rng, consistent)
+            if force == "x":
+                F = np.array([F[0], 0.0, 0.0])
+            elif force == "none":
+                F = np.zeros(3)
+            tau = rng.uniform(0.5005, 1.5)
+            got = hc_cell(hc, 27, 0, fast, f, rho, u, F, tau, mode)
+            ref = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, tau, mode)
+            assert np.abs(got - ref).max() < 1e-15
+        # the third-order oracle differs (the extra orders are seen)
+    ref3 = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, tau, mode)
+    assert np.abs(got - ref3).max() > 1e-10
+
+
+def test_sixth_order_cell_fp32(hc):
+    rng = np.random.default_rng(40)
+    with oracle.rr_order(6):
+        for _ in range(50):
+            f, rho, u, F = random_cell(oracle.D3Q27_CUDA, rng, True)
+            got = hc_cell(hc, 27, 1, 7, f, rho, u, F, 0.53, 0)
+            ref = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, 0.53, 0)
+            assert np.abs(got - ref).max() < 1e-7

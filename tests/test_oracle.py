@@ -440,3 +440,112 @@ def test_bgk_channel_step_conserves_mass():
     m0 = g.f.sum()
     g.step_bgk(20, 0.56)
     assert abs(g.f.sum() - m0) < 1e-14
+
+
+# ---------------------------------------------------------------- RR terms of order 4-6
+def product_form(c, u):
+    """Per-axis tables of the D3Q27 full-order equilibrium at the lattice
+    velocities: P = 1 + 3uc + 9/2 u^2 c^2 - 3/2 u^2, T1 = 3uc, T2 = 3/2 u^2 (3c^2 - 1),
+    h = 3c^2 - 1 (scaled second Hermite polynomial), D = c + (3c^2 - 1) u
+    (derivative of P in the scaled velocity, divided by sqrt(3))."""
+    c = c.astype(float)
+    P = 1 + 3 * u * c + 4.5 * u * u * c * c - 1.5 * u * u
+    T1 = 3 * u * c
+    T2 = 1.5 * u * u * (3 * c * c - 1)
+    h = 3 * c * c - 1
+    D = c + (3 * c * c - 1) * u
+    return P, T1, T2, h, D
+
+
+def sixth_order_cell(f, rho, u, F, tau, mode, c, w):
+    """The RR collision (mode 0) / RRBC (mode 1) with all Hermite orders D3Q27
+    carries, from the generating function G = prod_a P_a: feq = w rho G, the
+    non-equilibrium part w (1/2) sum_ab A_ab d2G/du~_a du~_b, the force term on
+    the equilibrium truncated after fourth order."""
+    P, T1, T2, h, D = product_form(c, u)
+    x, y, z = 0, 1, 2
+    feq6 = w * rho * P[:, x] * P[:, y] * P[:, z]
+    q4 = (P[:, x] * P[:, y] * P[:, z] - T2[:, x] * T2[:, y] * T1[:, z] - T2[:, x] * T1[:, y] * T2[:, z]
+          - T1[:, x] * T2[:, y] * T2[:, z] - T2[:, x] * T2[:, y] * T2[:, z])
+    feq4 = w * rho * q4
+    Fbar = 1.5 * (c @ F - u @ F)
+    g = f + Fbar * feq4
+    H2 = 3.0 * np.einsum("ia,ib->iab", c, c) - np.eye(3)[None]
+    A = np.einsum("iab,i->ab", H2, g) - 3.0 * rho * np.outer(u, u)
+    fneq = w * (0.5 * (A[x, x] * h[:, x] * P[:, y] * P[:, z] + A[y, y] * P[:, x] * h[:, y] * P[:, z]
+                       + A[z, z] * P[:, x] * P[:, y] * h[:, z])
+                + 3.0 * (A[x, y] * D[:, x] * D[:, y] * P[:, z] + A[x, z] * D[:, x] * P[:, y] * D[:, z]
+                         + A[y, z] * P[:, x] * D[:, y] * D[:, z]))
+    if mode == 0:
+        return feq6 + (1 - 1 / tau) * fneq + Fbar * feq4
+    return feq6 + fneq - Fbar * feq4
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN])
+def test_full_hermite_basis_orthogonal(kind):
+    """All 27 tensor Hermite polynomials of D3Q27 (orders 0-6): sum_i w H_m H_n =
+    delta_mn prod(a!) -- in particular sum_i w H222 = 0, which the reference's
+    commented H222 (initial.cu:127-129: + c_y^2 c_z^2 / cs^4) violates."""
+    c, w, _ = oracle.lattice(kind)
+    H = np.column_stack([np.ones(len(w)), oracle.hermite(kind), oracle.hermite_high(kind)])
+    orders = ["000", "100", "010", "001", "110", "101", "011", "200", "020", "002", "210", "201", "021", "120",
+              "102", "012", "111", "220", "202", "022", "211", "121", "112", "221", "212", "122", "222"]
+    norms = np.array([np.prod([2.0 if d == "2" else 1.0 for d in o]) for o in orders])
+    G = np.einsum("i,im,in->mn", w, H, H)
+    assert np.allclose(G, np.diag(norms), atol=1e-13)
+    # each table is the product of the 1D polynomials 1, c/cs, c^2/cs^2 - 1
+    one_d = [lambda v: np.ones_like(v, dtype=float), lambda v: v / CS, lambda v: v * v / CS ** 2 - 1]
+    for k, o in enumerate(orders):
+        ref = one_d[int(o[0])](c[:, 0]) * one_d[int(o[1])](c[:, 1]) * one_d[int(o[2])](c[:, 2])
+        assert np.allclose(H[:, k], ref, atol=1e-13), o
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN])
+def test_sixth_order_equilibrium_is_the_product_form(kind):
+    rng = np.random.default_rng(11)
+    c, w, _ = oracle.lattice(kind)
+    with oracle.rr_order(6):
+        for _ in range(20):
+            rho = 1 + 0.03 * rng.standard_normal()
+            u = 0.15 * rng.uniform(-1, 1, 3)
+            P = product_form(c, u)[0]
+            assert np.allclose(oracle.feq(kind, rho, *u), w * rho * P.prod(axis=1), rtol=1e-14, atol=1e-17)
+    # the third-order equilibrium differs by the fourth-order terms (the test sees them)
+    rho, u = 1.0, np.array([0.1, -0.08, 0.06])
+    diff = oracle.feq(kind, rho, *u) - w * rho * product_form(c, u)[0].prod(axis=1)
+    assert np.abs(diff).max() > 1e-6
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_sixth_order_collision_matches_generating_function(kind, mode):
+    rng = np.random.default_rng(12 + mode)
+    with oracle.rr_order(6):
+        for _ in range(20):
+            F = 2e-4 * rng.standard_normal(3)
+            f, rho, u, c, w = consistent_cell(kind, rng, F)
+            f = f + 5e-3 * w * rng.standard_normal(len(w))    # a sizeable non-equilibrium part
+            tau = rng.uniform(0.51, 1.5)
+            out = oracle.rr_cell(kind, f, rho, *u, F, tau, mode)
+            ref = sixth_order_cell(f, rho, u, F, tau, mode, c, w)
+            assert np.abs(out - ref).max() < 1e-15
+    # at third order the same inputs give a different state (so the test sees the extra terms)
+    out3 = oracle.rr_cell(kind, f, rho, *u, F, tau, mode)
+    assert np.abs(out3 - ref).max() > 1e-9
+
+
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q27_FORTRAN])
+def test_sixth_order_collision_conservation(kind):
+    rng = np.random.default_rng(13)
+    with oracle.rr_order(6):
+        for _ in range(20):
+            F = 1e-4 * rng.standard_normal(3)
+            f, rho, u, c, w = consistent_cell(kind, rng, F)
+            out = oracle.rr_cell(kind, f, rho, *u, F, rng.uniform(0.51, 2.0), 0)
+            assert abs(out.sum() - f.sum()) < 1e-14
+            assert np.allclose(out @ c, f @ c + 0.5 * np.asarray(F) * (1 + rho), atol=1e-15)
+            # tau = 1: equilibrium plus the force term, whatever f was
+            feq = oracle.feq(kind, rho, *u)
+            one = oracle.rr_cell(kind, f, rho, *u, (0.0, 0.0, 0.0), 1.0, 0)
+            assert np.allclose(one, feq, atol=1e-15)
+    assert oracle.get_rr_order() == 3
