@@ -485,7 +485,7 @@ def sixth_order_cell(f, rho, u, F, tau, mode, c, w):
 def test_full_hermite_basis_orthogonal(kind):
     """All 27 tensor Hermite polynomials of D3Q27 (orders 0-6): sum_i w H_m H_n =
     delta_mn prod(a!) -- in particular sum_i w H222 = 0, which the reference's
-    commented H222 (initial.cu:127-129: + c_y^2 c_z^2 / cs^4) violates."""
+    commented H222 (initial.cu:128-130: + c_y^2 c_z^2 / cs^4) violates."""
     c, w, _ = oracle.lattice(kind)
     H = np.column_stack([np.ones(len(w)), oracle.hermite(kind), oracle.hermite_high(kind)])
     orders = ["000", "100", "010", "001", "110", "101", "011", "200", "020", "002", "210", "201", "021", "120",
