@@ -93,7 +93,9 @@ typedef struct rrlbm_params {
     int32_t storage;            /* rrlbm_storage of the fused populations */
     int32_t wall_x, wall_y, wall_z; /* 1: half-way bounce-back walls on both faces of that
                                    axis (channel: wall_y); 0: periodic */
-    int32_t wall_rr;            /* 1: RR regularisation of wall-incoming populations (RRBC) */
+    int32_t wall_rr;            /* 1: RR regularisation of wall-incoming populations (RRBC);
+                                   2: the finite-difference reconstruction FDBC in its place
+                                   (D3Q27, y walls only, ly >= 4; drop-in kernels only) */
     int32_t lid_face;           /* -1 none, else face 2*axis + side (side 0 low, 1 high)
                                    moving with lid_u (cavity) */
     double tau;                 /* d_tau = 3 visc + 1/2 */
@@ -196,10 +198,17 @@ int rrlbm_collision_bgk(rrlbm_ctx* ctx);
 /* RRBC, BC.cu:4-126 (needs rrlbm_swap_f to have run: re-imposed bounce-back
  * values at multi-face cells come from f_temp). */
 int rrlbm_rrbc(rrlbm_ctx* ctx);
+/* FDBC, BC.cu:129-309: at the y-wall rows, velocity extrapolated from the two
+ * next rows, strain rate by finite differences, density from the known
+ * populations, all populations rebuilt as feq + fneq - Fbar feq (the
+ * alternative to RRBC at main.cu:76-77).  The wall velocities are all
+ * extrapolated before any difference is taken (the reference's single kernel
+ * races there).  D3Q27, y walls only, ly >= 4, third order. */
+int rrlbm_fdbc(rrlbm_ctx* ctx);
 /* macrovar_BN, macrovar.cu:33-62. */
 int rrlbm_macrovar_bn(rrlbm_ctx* ctx);
 /* nsteps of main.cu:50-84 with the reference kernels: [LES_WALE] ->
- * collision_RR -> streaming -> BB -> copy -> macrovar -> [RRBC -> macrovar_BN]
+ * collision_RR -> streaming -> BB -> copy -> macrovar -> [RRBC (FDBC with wall_rr 2) -> macrovar_BN]
  * (BGK model: BGK/main.cu:49-70, collision_BGK -> streaming -> BB -> copy ->
  * macrovar).  The force fields are used as they are. */
 int rrlbm_step_reference(rrlbm_ctx* ctx, int nsteps);

@@ -197,9 +197,15 @@ class Fields:
                         _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz), *[_I(w) for w in walls], _I(lid_face),
                         _p(lu))
 
+    def fdbc(self, tau, use_les=False):
+        """FDBC, BC.cu:129-309 (y walls; wall velocities extrapolated first)."""
+        assert self.kind == D3Q27_CUDA
+        load().orc_fdbc(*self.dims, _p(self.f), _p(self.fx), _p(self.fy), _p(self.fz), _p(self.ux), _p(self.uy),
+                        _p(self.uz), _p(self.taut) if use_les else None, ctypes.c_double(tau))
+
     def step(self, nsteps, tau, use_les=False, cw=0.5, walls=(0, 1, 0), wall_rr=1, lid_face=-1,
              lid_u=(0.0, 0.0, 0.0)):
-        """main.cu:50-84 order, nsteps times."""
+        """main.cu:50-84 order, nsteps times (wall_rr 2: FDBC in place of RRBC)."""
         lu = np.array(lid_u, dtype=np.float64)
         load().orc_step_channel(self.kind, *self.dims, _I(nsteps), _p(self.f), _p(self.ftmp), _p(self.fx),
                                 _p(self.fy), _p(self.fz), _p(self.rho), _p(self.ux), _p(self.uy), _p(self.uz),

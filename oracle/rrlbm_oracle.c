@@ -39,6 +39,12 @@
  *                                the non-equilibrium part against its
  *                                generating-function form (second derivatives
  *                                of that product), mass and momentum balance
+ *   orc_fdbc                  -> closed forms: uniform flow gives the
+ *                                equilibrium at 7/15 u with the known-sum
+ *                                density; linear and parabolic profiles are
+ *                                extrapolated exactly; the reconstructed
+ *                                second moment is the Chapman-Enskog stress
+ *                                -2/3 rho tau S; idempotence
  *   orc_collision_bgk,        -> mass conservation, momentum change = F,
  *   orc_macrovar_bgk             the rest state as a fixed point, the
  *                                second-moment relaxation rate 1 - 1/tau
@@ -500,6 +506,90 @@ void orc_rrbc(int kind, int lx, int ly, int lz, double* f, const double* fpost, 
             }
 }
 
+/* FDBC, BC.cu:129-309: finite-difference wall reconstruction at the y walls
+ * (the channel's alternative to RRBC, main.cu:76-77).  At each wall row
+ * (iy = 0 and LY-1) the velocity is extrapolated from the two next rows to
+ * the wall node (quadratic through u = 0 on the half-way wall,
+ * BC.cu:149-161), the strain rate is taken by finite differences (one-sided
+ * second order in y, central in x and z with periodic wrap, BC.cu:163-196),
+ * the density from the ratio of the known populations' sum (c_y pointing into
+ * the wall) to the same sum of the reconstruction at rho = 1
+ * (BC.cu:203-265), and all populations are replaced by
+ * feq + fneq(-rho tau_eff S) - Fbar feq at that density (BC.cu:268-308).
+ * The reference kernel extrapolates and differentiates in one pass, so its x
+ * and z differences read neighbours' wall velocities that other threads are
+ * overwriting; here every wall velocity is extrapolated first (phase 1) and
+ * the differences use the extrapolated values (phase 2).  Third order,
+ * D3Q27 channel ordering; ly >= 4.  taut may be NULL. */
+void orc_fdbc(int lx, int ly, int lz, double* f, const double* fx, const double* fy, const double* fz, double* ux,
+              double* uy, double* uz, const double* taut, double tau) {
+    orc_lat L;
+    lat_build(ORC_D3Q27_CUDA, &L);
+    const long N = (long)lx * ly * lz;
+    /* phase 1: wall velocities (BC.cu:149-161) */
+    for (int iz = 0; iz < lz; ++iz)
+        for (int side = 0; side < 2; ++side)
+            for (int ix = 0; ix < lx; ++ix) {
+                int iy = side ? ly - 1 : 0, d = side ? -1 : 1;
+                long idx = cidx(ix, iy, iz, lx, ly);
+                long i1 = cidx(ix, iy + d, iz, lx, ly), i2 = cidx(ix, iy + 2 * d, iz, lx, ly);
+                ux[idx] = 2.0 / 3.0 * ux[i1] - 1.0 / 5.0 * ux[i2];
+                uy[idx] = 2.0 / 3.0 * uy[i1] - 1.0 / 5.0 * uy[i2];
+                uz[idx] = 2.0 / 3.0 * uz[i1] - 1.0 / 5.0 * uz[i2];
+            }
+    /* phase 2: reconstruction */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = 0; iz < lz; ++iz)
+        for (int side = 0; side < 2; ++side)
+            for (int ix = 0; ix < lx; ++ix) {
+                int iy = side ? ly - 1 : 0, d = side ? -1 : 1;
+                long idx = cidx(ix, iy, iz, lx, ly);
+                long i1 = cidx(ix, iy + d, iz, lx, ly), i2 = cidx(ix, iy + 2 * d, iz, lx, ly);
+                double tau_eff = tau + (taut ? taut[idx] : 0.0);
+                /* BC.cu:167-180: (-3 u0 + 4 u1 - u2)/2 at the bottom, (3 u0 - 4 u1 + u2)/2 at the top */
+                double sg = side ? -1.0 : 1.0;
+                double dudy = sg * (-3.0 * ux[idx] + 4.0 * ux[i1] - ux[i2]) / 2.0;
+                double dvdy = sg * (-3.0 * uy[idx] + 4.0 * uy[i1] - uy[i2]) / 2.0;
+                double dwdy = sg * (-3.0 * uz[idx] + 4.0 * uz[i1] - uz[i2]) / 2.0;
+                int ixp = (ix + 1) % lx, ixm = (ix - 1 + lx) % lx, izp = (iz + 1) % lz, izm = (iz - 1 + lz) % lz;
+                long xp = cidx(ixp, iy, iz, lx, ly), xm = cidx(ixm, iy, iz, lx, ly);
+                long zp = cidx(ix, iy, izp, lx, ly), zm = cidx(ix, iy, izm, lx, ly);
+                double dudx = 0.5 * (ux[xp] - ux[xm]), dvdx = 0.5 * (uy[xp] - uy[xm]), dwdx = 0.5 * (uz[xp] - uz[xm]);
+                double dudz = 0.5 * (ux[zp] - ux[zm]), dvdz = 0.5 * (uy[zp] - uy[zm]), dwdz = 0.5 * (uz[zp] - uz[zm]);
+                double S110 = 0.5 * (dudy + dvdx), S101 = 0.5 * (dudz + dwdx), S011 = 0.5 * (dvdz + dwdy);
+                double S200 = dudx, S020 = dvdy, S002 = dwdz;
+                double u9 = ux[idx], v9 = uy[idx], w9 = uz[idx];
+                double Fbar[27];
+                fbar_cell(&L, u9, v9, w9, fx[idx], fy[idx], fz[idx], Fbar);
+                double uu = u9 / ORC_CS, vv = v9 / ORC_CS, ww = w9 / ORC_CS;
+                orc_a a;
+                eq_coeffs(uu, vv, ww, &a);
+                /* the reconstruction per unit density: feq(1) + fneq(-tau S) - Fbar feq(1), BC.cu:198-265 */
+                orc_a n;
+                n.a110 = -tau_eff * S110; n.a101 = -tau_eff * S101; n.a011 = -tau_eff * S011;
+                n.a200 = -tau_eff * S200; n.a020 = -tau_eff * S020; n.a002 = -tau_eff * S002;
+                n.a210 = n.a200 * vv + 2. * uu * n.a110;
+                n.a201 = n.a200 * ww + 2. * uu * n.a101;
+                n.a120 = n.a020 * uu + 2. * vv * n.a110;
+                n.a021 = n.a020 * ww + 2. * vv * n.a011;
+                n.a102 = n.a002 * uu + 2. * ww * n.a101;
+                n.a012 = n.a002 * vv + 2. * ww * n.a011;
+                n.a111 = n.a110 * ww + n.a101 * vv + n.a011 * uu;
+                double g1[27], sum_ps = 0.0, sum_in = 0.0;
+                for (int ip = 0; ip < 27; ++ip) {
+                    double fir = L.H100[ip] * uu + L.H010[ip] * vv + L.H001[ip] * ww;
+                    double feq1 = L.w[ip] * (1.0 + fir + 0.5 * (second_sum(&L, ip, &a) + third_sum(&L, ip, &a)));
+                    double fneq1 = L.w[ip] * (second_sum(&L, ip, &n) + third_sum(&L, ip, &n));
+                    g1[ip] = feq1 + fneq1 - Fbar[ip] * feq1;
+                    int known = side ? L.cy[ip] > 0 : L.cy[ip] < 0;
+                    if (known) { sum_ps += f[ip * N + idx]; sum_in += g1[ip]; }
+                }
+                /* every term of the reconstruction is proportional to rho (BC.cu:268-308) */
+                double rho_l = sum_ps / sum_in;
+                for (int ip = 0; ip < 27; ++ip) f[ip * N + idx] = rho_l * g1[ip];
+            }
+}
+
 /* ------------------------------------------------------------------ */
 /* WALE, forcing, initialisation                                       */
 /* ------------------------------------------------------------------ */
@@ -641,7 +731,11 @@ void orc_step_channel(int kind, int lx, int ly, int lz, int nsteps, double* f, d
         /* after the copy f <- ftmp, ftmp keeps the post-collision populations */
         for (long i = 0; i < (long)Q * N; ++i) { double t = f[i]; f[i] = ftmp[i]; ftmp[i] = t; }
         orc_macrovar(kind, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 0, 0, 0, 0);
-        if (wall_rr && (wall_x || wall_y || wall_z)) {
+        if (wall_rr == 2 && wall_y) {
+            /* FDBC in place of RRBC (main.cu:76-77), then macrovar_BN */
+            orc_fdbc(lx, ly, lz, f, fx, fy, fz, ux, uy, uz, use_les ? taut : NULL, tau);
+            orc_macrovar(kind, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 1, wall_x, wall_y, wall_z);
+        } else if (wall_rr && (wall_x || wall_y || wall_z)) {
             orc_rrbc(kind, lx, ly, lz, f, ftmp, fx, fy, fz, rho, ux, uy, uz, wall_x, wall_y, wall_z, lid_face, lid_u);
             orc_macrovar(kind, lx, ly, lz, f, fx, fy, fz, rho, ux, uy, uz, 1, wall_x, wall_y, wall_z);
         }

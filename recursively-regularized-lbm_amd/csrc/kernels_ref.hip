@@ -222,6 +222,92 @@ __global__ __launch_bounds__(kBlock) void k_rrbc(RefFields r, const double* fpos
   });
 }
 
+// FDBC (BC.cu:129-309), the finite-difference wall reconstruction at the y
+// walls, in two passes over the 2*LX*LZ wall cells (t -> x = t % LX, then the
+// side, then z).  Pass 1 extrapolates every wall velocity from the two next
+// rows (BC.cu:149-161, quadratic through u = 0 on the half-way wall); pass 2
+// differentiates (one-sided second order in y, BC.cu:167-180; central in x
+// and z over the extrapolated wall row, BC.cu:182-196) and replaces all
+// populations by rho_l (feq + fneq(-tau_eff S) - Fbar feq)(rho = 1), rho_l
+// keeping the sum of the known populations (c_y pointing into the wall,
+// BC.cu:203-265).  The reference does both in one kernel, where the x/z
+// differences race with the neighbours' extrapolation; the two passes give
+// every difference the extrapolated values.
+struct WallCell {
+  uint32_t x, y, z, y1, y2;
+  bool top;
+};
+RRLBM_INLINE __device__ WallCell wall_cell(uint32_t t, const Grid& g) {
+  WallCell w;
+  w.x = t % g.lx;
+  const uint32_t r = t / g.lx;
+  w.top = (r & 1u) != 0;
+  w.z = r >> 1;
+  w.y = w.top ? g.ly - 1 : 0;
+  w.y1 = w.top ? g.ly - 2 : 1;
+  w.y2 = w.top ? g.ly - 3 : 2;
+  return w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fdbc_velocity(RefFields r, Grid g) {
+  const uint32_t t = tid();
+  if (t >= 2 * g.lx * g.lz) return;
+  const WallCell w = wall_cell(t, g);
+  const uint32_t c = lin(w.x, w.y, w.z, g), c1 = lin(w.x, w.y1, w.z, g), c2 = lin(w.x, w.y2, w.z, g);
+  r.ux[c] = 2.0 / 3.0 * r.ux[c1] - 1.0 / 5.0 * r.ux[c2];
+  r.uy[c] = 2.0 / 3.0 * r.uy[c1] - 1.0 / 5.0 * r.uy[c2];
+  r.uz[c] = 2.0 / 3.0 * r.uz[c1] - 1.0 / 5.0 * r.uz[c2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fdbc_reconstruct(RefFields r, Grid g, double tau, int use_les) {
+  const uint32_t t = tid();
+  if (t >= 2 * g.lx * g.lz) return;
+  const WallCell w = wall_cell(t, g);
+  const uint32_t c = lin(w.x, w.y, w.z, g), c1 = lin(w.x, w.y1, w.z, g), c2 = lin(w.x, w.y2, w.z, g);
+  const uint32_t xp = lin(w.x + 1 == g.lx ? 0 : w.x + 1, w.y, w.z, g);
+  const uint32_t xm = lin(w.x == 0 ? g.lx - 1 : w.x - 1, w.y, w.z, g);
+  const uint32_t zp = lin(w.x, w.y, w.z + 1 == g.lz ? 0 : w.z + 1, g);
+  const uint32_t zm = lin(w.x, w.y, w.z == 0 ? g.lz - 1 : w.z - 1, g);
+  const double sg = w.top ? -1.0 : 1.0;
+  auto ddy = [&](const double* u) { return sg * (-3.0 * u[c] + 4.0 * u[c1] - u[c2]) * 0.5; };
+  auto ddx = [&](const double* u) { return 0.5 * (u[xp] - u[xm]); };
+  auto ddz = [&](const double* u) { return 0.5 * (u[zp] - u[zm]); };
+  const double tau_eff = tau + (use_les ? r.taut[c] : 0.0);
+  // the reconstruction per unit density: feq + fneq - Fbar feq with the
+  // non-equilibrium Hermite coefficients -tau_eff S used without the 1/2 of
+  // the collision's fneq, i.e. the moment-space form with N = -(2/3) tau_eff S
+  const double k = -2.0 / 3.0 * tau_eff;
+  NeqTensor<double> N;
+  N.xx = k * ddx(r.ux);
+  N.yy = k * ddy(r.uy);
+  N.zz = k * ddz(r.uz);
+  N.xy = k * 0.5 * (ddy(r.ux) + ddx(r.uy));
+  N.xz = k * 0.5 * (ddz(r.ux) + ddx(r.uz));
+  N.yz = k * 0.5 * (ddz(r.uy) + ddy(r.uz));
+  Macro<double> q;
+  q.rho = 1.0; q.dr = 0.0;
+  q.ux = r.ux[c]; q.uy = r.uy[c]; q.uz = r.uz[c];
+  using L = D3Q27;
+  double g1[27];
+  rr_post_q27<double, true>(q, N, r.fx[c], r.fy[c], r.fz[c], 1.0, -1.0, 1.0, g1);
+  double sum_ps = 0.0, sum_in = 0.0;
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    g1[i] += L::w(i);
+    if constexpr (L::cy(i) != 0) {
+      if ((L::cy(i) > 0) == w.top) {
+        sum_ps += r.f[(size_t)i * g.n + c];
+        sum_in += g1[i];
+      }
+    }
+  });
+  const double rho_l = sum_ps / sum_in;
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    r.f[(size_t)i * g.n + c] = rho_l * g1[i];
+  });
+}
+
 // LES_WALE (LES.cu:4-98): central differences, one-sided at walled faces.
 __global__ __launch_bounds__(kBlock) void k_les_wale(RefFields r, Grid g, double cw) {
   const uint32_t cell = tid();
@@ -576,6 +662,15 @@ hipError_t ref_collision_bgk(const RefFields& r, const Grid& g, double tau, hipS
 hipError_t ref_rrbc(int q, const RefFields& r, const double* fpost, const Grid& g, hipStream_t s) {
   if (q == 27) RRLBM_LAUNCH(k_rrbc<D3Q27>, g, s, r, fpost, g);
   RRLBM_LAUNCH(k_rrbc<D3Q19>, g, s, r, fpost, g);
+}
+hipError_t ref_fdbc(const RefFields& r, const Grid& g, double tau, int use_les, hipStream_t s) {
+  const uint32_t nw = 2 * g.lx * g.lz;
+  const dim3 grid((nw + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_fdbc_velocity, grid, dim3(kBlock), 0, s, r, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fdbc_reconstruct, grid, dim3(kBlock), 0, s, r, g, tau, use_les);
+  return hipGetLastError();
 }
 hipError_t ref_les_wale(const RefFields& r, const Grid& g, double cw, hipStream_t s) {
   RRLBM_LAUNCH(k_les_wale, g, s, r, g, cw);

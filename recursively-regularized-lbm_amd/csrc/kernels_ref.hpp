@@ -44,6 +44,8 @@ hipError_t ref_macrovar(int q, const RefFields& r, const Grid& g, int wall_only,
 // collision_BGK (BGK/collision.cu:3-33), D3Q27
 hipError_t ref_collision_bgk(const RefFields& r, const Grid& g, double tau, hipStream_t s);
 hipError_t ref_rrbc(int q, const RefFields& r, const double* fpost, const Grid& g, hipStream_t s);
+// FDBC (BC.cu:129-309) at the y walls, D3Q27, ly >= 4
+hipError_t ref_fdbc(const RefFields& r, const Grid& g, double tau, int use_les, hipStream_t s);
 hipError_t ref_les_wale(const RefFields& r, const Grid& g, double cw, hipStream_t s);
 hipError_t ref_forcing(const RefFields& r, const Grid& g, double fx, double fy, double fz, hipStream_t s);
 hipError_t ref_forcing_gp(const RefFields& r, const Grid& g, double force_in_x, int istep, double phase,

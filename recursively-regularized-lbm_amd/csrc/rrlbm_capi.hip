@@ -410,6 +410,11 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     return fail(nullptr, RRLBM_E_INVALID, "collision_model must be 0 (RR) or 1 (BGK)");
   if (p.collision_model == RRLBM_COLLISION_BGK && (p.npop != 27 || p.use_les || p.wall_rr))
     return fail(nullptr, RRLBM_E_UNSUPPORTED, "the BGK baseline is D3Q27 without LES and without RR wall regularisation");
+  if (p.wall_rr < 0 || p.wall_rr > 2) return fail(nullptr, RRLBM_E_INVALID, "wall_rr must be 0, 1 (RRBC) or 2 (FDBC)");
+  if (p.wall_rr == 2 && (p.npop != 27 || !p.wall_y || p.wall_x || p.wall_z || p.lid_face >= 0 || p.ly < 4 ||
+                         p.collision_model != RRLBM_COLLISION_RR || p.rr_order == 6))
+    return fail(nullptr, RRLBM_E_UNSUPPORTED,
+                "FDBC (wall_rr 2) is the D3Q27 third-order RR channel's y-wall treatment (y walls only, ly >= 4)");
   if (p.rr_order == 0) p.rr_order = 3;
   if (p.rr_order != 3 && p.rr_order != 6) return fail(nullptr, RRLBM_E_INVALID, "rr_order must be 0, 3 or 6");
   if (p.rr_order == 6 && (p.collision_model != RRLBM_COLLISION_RR || p.npop != 27 || p.use_les))
@@ -625,6 +630,16 @@ int rrlbm_rrbc(rrlbm_ctx* c) {
   return RRLBM_OK;
 }
 
+int rrlbm_fdbc(rrlbm_ctx* c) {
+  NEED_REF(c);
+  if (c->q != 27 || !c->p.wall_y || c->p.wall_x || c->p.wall_z || c->p.lid_face >= 0 || c->p.ly < 4 ||
+      is_bgk(c->p) || is_rr6(c->p))
+    return fail(c, RRLBM_E_UNSUPPORTED,
+                "FDBC is the D3Q27 third-order RR channel's y-wall treatment (y walls only, ly >= 4)");
+  CK(c, ref_fdbc(c->ref, c->grid, c->p.tau, c->p.use_les, c->stream));
+  return RRLBM_OK;
+}
+
 int rrlbm_macrovar_bn(rrlbm_ctx* c) {
   NEED_REF(c);
   if (c->grid.walls) CK(c, ref_macrovar(c->q, c->ref, c->grid, 1, c->stream, is_bgk(c->p)));
@@ -644,7 +659,8 @@ int rrlbm_step_reference(rrlbm_ctx* c, int nsteps) {
     if ((rc = rrlbm_swap_f(c))) return rc;
     if ((rc = rrlbm_macrovar(c))) return rc;
     if (c->grid.walls && c->p.wall_rr) {
-      if ((rc = rrlbm_rrbc(c))) return rc;
+      // main.cu:74-77: RRBC, or FDBC in its place (wall_rr = 2)
+      if ((rc = c->p.wall_rr == 2 ? rrlbm_fdbc(c) : rrlbm_rrbc(c))) return rc;
       if ((rc = rrlbm_macrovar_bn(c))) return rc;
     }
   }
@@ -702,6 +718,8 @@ int rrlbm_init_equilibrium(rrlbm_ctx* c) {
 
 int rrlbm_fused_pack(rrlbm_ctx* c) {
   NEED_REF(c);
+  if (c->p.wall_rr == 2)
+    return fail(c, RRLBM_E_UNSUPPORTED, "the FDBC wall treatment runs through the drop-in kernels only");
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
   CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->lay, c->p.tau, c->p.use_les, c->p.force[0],
                  c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream, is_bgk(c->p)));
@@ -990,6 +1008,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   if (trial_steps < 1 || trial_steps > 10000) return fail(c, RRLBM_E_INVALID, "trial_steps must be in [1, 10000]");
   if (c->packed)
     return fail(c, RRLBM_E_STATE, "rrlbm_autotune runs before rrlbm_fused_pack (it overwrites the fused buffers)");
+  if (c->p.wall_rr == 2)
+    return fail(c, RRLBM_E_UNSUPPORTED, "the FDBC wall treatment runs through the drop-in kernels only");
   memset(out, 0, sizeof *out);
   const std::string key = tune_key(c->p);
   {

@@ -102,6 +102,7 @@ SIGNATURES = {
     "rrlbm_macrovar": [_P],
     "rrlbm_rrbc": [_P],
     "rrlbm_macrovar_bn": [_P],
+    "rrlbm_fdbc": [_P],
     "rrlbm_step_reference": [_P, _I],
     "rrlbm_init_rest": [_P],
     "rrlbm_init_taylor_green_2d": [_P, ctypes.c_double],

@@ -190,6 +190,10 @@ class Solver:
         """RRBC, BC.cu:4-126."""
         self._call("rrlbm_rrbc")
 
+    def FDBC(self):
+        """FDBC, BC.cu:129-309 (wall velocities extrapolated before the differences)."""
+        self._call("rrlbm_fdbc")
+
     def macrovar_BN(self):
         """macrovar_BN, macrovar.cu:33-62."""
         self._call("rrlbm_macrovar_bn")

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import maxabs, random_state, upload_state
+from helpers import channel_state, maxabs, random_state, upload_state
 
 pytestmark = pytest.mark.gpu
 
@@ -149,3 +149,46 @@ def test_step_reference_sequence(rr):
     assert maxabs(got_f, g.f) < 1e-12
     for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
         assert maxabs(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("les", [False, True])
+@pytest.mark.parametrize("lx", [12, 64])
+def test_fdbc_matches_oracle(rr, les, lx):
+    """FDBC (BC.cu:129-309) alone and inside the reference-order step
+    (wall_rr = 2), against the oracle's restatement."""
+    g = channel_state(oracle.D3Q27_CUDA, lx, 10, 6, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 1e-6, 0))
+    g.taut[:] = 0.02 * np.random.default_rng(1).uniform(0, 1, g.n)
+    tau = 0.5 + 3 * 0.002
+    with rr.Solver(lx=lx, ly=10, lz=6, storage=0, tau=tau, force=(2e-5, 1e-6, 0), wall_y=1, wall_rr=2,
+                   use_les=int(les)) as s:
+        upload_state(s, g)
+        s.FDBC()
+        f1 = s.download(rr.Field.F)
+        u1 = s.macroscopic()[1:]
+        s.step_reference(5)
+        f2 = s.download(rr.Field.F)
+        m2 = s.macroscopic()
+    g.fdbc(tau, use_les=les)
+    assert maxabs(f1, g.f) < 1e-14
+    for a, b in zip(u1, (g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-16
+    g.step(5, tau, use_les=les, walls=(0, 1, 0), wall_rr=2)
+    assert maxabs(f2, g.f) < 1e-13
+    for a, b in zip(m2, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-13
+
+
+def test_fdbc_parameter_checks(rr):
+    for bad in (dict(npop=19), dict(wall_x=1), dict(ly=3), dict(lid_face=2, lid_u=(0.01, 0, 0)), dict(rr_order=6)):
+        kw = dict(lx=8, ly=8, lz=4, wall_y=1, wall_rr=2)
+        kw.update(bad)
+        with pytest.raises(rr.RRLBMError) as ei:
+            rr.Solver(**kw)
+        assert ei.value.status == 6, bad
+    with rr.Solver(lx=8, ly=8, lz=4, wall_y=1, wall_rr=2) as s:
+        with pytest.raises(rr.RRLBMError) as ei:
+            s.fused_pack()
+        assert ei.value.status == 6
+    with rr.Solver(lx=8, ly=8, lz=4, wall_y=1, wall_rr=1) as s:
+        with pytest.raises(rr.RRLBMError):
+            rr.Solver(lx=8, ly=8, lz=4, wall_y=1, wall_rr=3)

@@ -549,3 +549,86 @@ def test_sixth_order_collision_conservation(kind):
             one = oracle.rr_cell(kind, f, rho, *u, (0.0, 0.0, 0.0), 1.0, 0)
             assert np.allclose(one, feq, atol=1e-15)
     assert oracle.get_rr_order() == 3
+
+
+# ---------------------------------------------------------------- FDBC
+def fdbc_field(lx, ly, lz, seed):
+    """Oracle Fields whose velocity is, in every (x, z) column, a quadratic in y
+    vanishing on both half-way walls (y = -1/2 and LY - 1/2), modulated
+    smoothly in x and z; returns the fields and the exact profile functions."""
+    rng = np.random.default_rng(seed)
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    amp = np.array([0.02, 0.004, -0.01])
+    ex, ez = 0.3, 0.2
+    X, Y, Z = np.meshgrid(np.arange(lx), np.arange(ly), np.arange(lz), indexing="xy")
+    X, Y, Z = (np.transpose(a, (2, 0, 1)).ravel() for a in (X, Y, Z))  # idx = (z*LY + y)*LX + x
+    mod = [1 + ex * np.sin(2 * np.pi * (X + k) / lx) + ez * np.cos(2 * np.pi * (Z + 2 * k) / lz) for k in range(3)]
+    prof = (Y + 0.5) * (ly - 0.5 - Y)
+    dprof = (ly - 1 - 2.0 * Y)
+    for k, arr in enumerate((g.ux, g.uy, g.uz)):
+        arr[:] = amp[k] * mod[k] * prof
+    _, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    g.f[:] = w[:, None] * (1 + 0.01 * rng.standard_normal((27, g.n)))
+    return g, amp, mod, prof, dprof, (X, Y, Z)
+
+
+def test_fdbc_closed_forms():
+    lx, ly, lz, tau = 8, 9, 6, 0.6
+    g, amp, mod, prof, dprof, (X, Y, Z) = fdbc_field(lx, ly, lz, 3)
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    H = oracle.hermite(oracle.D3Q27_CUDA)
+    fin = g.f.copy()
+    interior = (Y > 0) & (Y < ly - 1)
+    u_before = np.stack([g.ux, g.uy, g.uz]).copy()
+    g.fdbc(tau)
+    u = np.stack([g.ux, g.uy, g.uz])
+    exact = np.stack([amp[k] * mod[k] * prof for k in range(3)])
+    wall = ~interior
+    # quadratic extrapolation is exact for these profiles; the interior is untouched
+    assert np.abs(u[:, wall] - exact[:, wall]).max() < 1e-16
+    assert np.array_equal(u[:, interior], u_before[:, interior])
+    assert np.array_equal(g.f[:, interior], fin[:, interior])
+    # strain rate at the wall: exact y derivative (the one-sided difference is
+    # exact for quadratics), central differences in x and z of the wall values
+    def at(arr, x, y, z):
+        return arr[((z % lz) * ly + y) * lx + (x % lx)]
+    for idx in np.where(wall)[0]:
+        x, y, z = X[idx], Y[idx], Z[idx]
+        grad = np.zeros((3, 3))  # grad[a, b] = d u_a / d x_b
+        for a in range(3):
+            grad[a, 0] = 0.5 * (at(u[a], x + 1, y, z) - at(u[a], x - 1, y, z))
+            grad[a, 1] = amp[a] * mod[a][idx] * dprof[idx]
+            grad[a, 2] = 0.5 * (at(u[a], x, y, z + 1) - at(u[a], x, y, z - 1))
+        S = 0.5 * (grad + grad.T)
+        known = c[:, 1] < 0 if y == 0 else c[:, 1] > 0
+        fo, fi = g.f[:, idx], fin[:, idx]
+        # the density keeps the known populations' sum
+        assert abs(fo[known].sum() - fi[known].sum()) < 1e-15
+        rho = fo.sum()
+        uw = u[:, idx]
+        assert np.allclose(fo @ c, rho * uw, atol=1e-16)
+        # second moment: rho (I/3 + u u) + the Chapman-Enskog stress -2/3 rho tau S
+        P = np.einsum("i,ia,ib->ab", fo, c, c)
+        assert np.allclose(P, rho * (np.eye(3) / 3 + np.outer(uw, uw) - 2.0 / 3.0 * tau * S), atol=1e-16)
+        # third order: sum H210 f = rho (u~x^2 u~y + 2 A210), A210 = A200 u~y + 2 u~x A110, A = -tau S
+        ut = uw / CS
+        A200, A110 = -tau * S[0, 0], -tau * S[0, 1]
+        assert abs(fo @ H[:, 9] - rho * (ut[0] ** 2 * ut[1] + 2 * (A200 * ut[1] + 2 * ut[0] * A110))) < 1e-15
+    # idempotent: a second pass keeps the state
+    f1 = g.f.copy()
+    g.fdbc(tau)
+    assert np.abs(g.f - f1).max() < 1e-15
+
+
+def test_fdbc_force_term():
+    """With a body force the reconstruction carries -Fbar feq: its first moment
+    is rho (u_w - F/2)."""
+    g, amp, mod, prof, dprof, (X, Y, Z) = fdbc_field(6, 7, 4, 4)
+    F = np.array([2e-4, -1e-4, 5e-5])
+    g.set_force(F)
+    c, _, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    g.fdbc(0.7)
+    for idx in np.where((Y == 0) | (Y == 6))[0]:
+        fo = g.f[:, idx]
+        uw = np.array([g.ux[idx], g.uy[idx], g.uz[idx]])
+        assert np.allclose(fo @ c, fo.sum() * (uw - F / 2), atol=1e-16)
