@@ -95,7 +95,7 @@ typedef struct rrlbm_params {
                                    axis (channel: wall_y); 0: periodic */
     int32_t wall_rr;            /* 1: RR regularisation of wall-incoming populations (RRBC);
                                    2: the finite-difference reconstruction FDBC in its place
-                                   (D3Q27, y walls only, ly >= 4; drop-in kernels only) */
+                                   (D3Q27, y walls only, ly >= 4; the fused path without LES) */
     int32_t lid_face;           /* -1 none, else face 2*axis + side (side 0 low, 1 high)
                                    moving with lid_u (cavity) */
     double tau;                 /* d_tau = 3 visc + 1/2 */

@@ -35,8 +35,8 @@ RRLBM_HD RRLBM_INLINE uint32_t fastdiv(uint32_t n, const FastDiv& f) {
 enum Storage : int { kF64 = 0, kF32 = 1, kF16 = 2 };
 enum Mode : int { kStep = 0, kMacro = 1, kFinish = 2, kDiag = 3 };
 // collision operator of a fused kernel: third-order RR, BGK baseline, RR with
-// the fourth- to sixth-order terms
-enum FusedOp : int { kOpRR = 0, kOpBGK = 1, kOpRR6 = 2 };
+// the fourth- to sixth-order terms, third-order RR with the FDBC wall rows
+enum FusedOp : int { kOpRR = 0, kOpBGK = 1, kOpRR6 = 2, kOpRRFD = 3 };
 
 template <int S>
 struct StoreT;
@@ -109,6 +109,8 @@ struct FusedArgs {
   double* ref_uy;
   double* ref_uz;
   double* partials;     // diagnostics pass: per-workgroup FP64 partial sums
+  void* u_wall;         // FDBC: velocity of rows 1, 2, LY-2, LY-3 after this step's streaming,
+                        // [4][LZ][LX] x (ux, uy, uz) in the compute type
   uint32_t lx, ly, lz, n;
   FastDiv div_lx, div_lxly;
   uint32_t walls;       // bit 0: x faces, bit 1: y faces, bit 2: z faces
@@ -134,10 +136,13 @@ struct FusedArgs {
 // Host launchers (kernels_fused.hip)
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s);
-// The alternative operators op = kOpBGK / kOpRR6 (D3Q27, no LES): step, finish
-// and diagnostics modes.
+// The alternative operators op = kOpBGK / kOpRR6 / kOpRRFD (D3Q27, no LES):
+// step, finish and diagnostics modes.  kOpRRFD needs launch_fdbc_rows on the
+// same source buffer first.
 hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const FusedArgs& a, uint32_t grid,
                            hipStream_t s);
+// FDBC pre-pass: macroscopic velocity of rows 1, 2, LY-2, LY-3 into a.u_wall.
+hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list

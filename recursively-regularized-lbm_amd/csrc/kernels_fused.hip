@@ -257,6 +257,97 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
   });
 }
 
+// FDBC at a y-wall cell (BC.cu:129-309, the drop-in k_fdbc_* in one place):
+// wall velocities extrapolated from rows 1, 2 (or LY-2, LY-3) of a.u_wall for
+// this cell and its x and z neighbours, strain rate by finite differences,
+// then every population replaced by the reconstruction at the density that
+// keeps the known populations' sum (c_y pointing into the wall).  h: the
+// cell's gathered shifted populations, replaced in place.  With the
+// reconstruction per unit density g = w + r (r from rr_post_q27 with rho = 1),
+// rho_l - 1 = (sum_k h - sum_k r) / (W + sum_k r), W = sum_k w = 1/6, and the
+// new h = r + (rho_l - 1) g: only small quantities enter.
+template <int S, int FM>
+RRLBM_INLINE __device__ void fdbc_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[27], uint32_t x,
+                                            uint32_t y, uint32_t z) {
+  using C = typename StoreT<S>::C;
+  using L = D3Q27;
+  const bool top = y != 0;
+  const uint32_t k1 = top ? 2u : 0u, k2 = top ? 3u : 1u;
+  const Vel3<C>* U = reinterpret_cast<const Vel3<C>*>(a.u_wall);
+  auto row = [&](uint32_t k, uint32_t xx, uint32_t zz) { return U[(k * a.lz + zz) * a.lx + xx]; };
+  auto wall = [&](uint32_t xx, uint32_t zz) {
+    const Vel3<C> u1 = row(k1, xx, zz), u2 = row(k2, xx, zz);
+    const C c1 = C(2.0 / 3.0), c2 = C(1.0 / 5.0);
+    return Vel3<C>{c1 * u1.x - c2 * u2.x, c1 * u1.y - c2 * u2.y, c1 * u1.z - c2 * u2.z};
+  };
+  const uint32_t xp = x + 1 == a.lx ? 0u : x + 1, xm = x == 0 ? a.lx - 1 : x - 1;
+  const uint32_t zp = z + 1 == a.lz ? 0u : z + 1, zm = z == 0 ? a.lz - 1 : z - 1;
+  const Vel3<C> u0 = wall(x, z), uxp = wall(xp, z), uxm = wall(xm, z), uzp = wall(x, zp), uzm = wall(x, zm);
+  const Vel3<C> u1 = row(k1, x, z), u2 = row(k2, x, z);
+  const C hs = top ? C(-0.5) : C(0.5);
+  const C dudy = hs * (C(-3) * u0.x + C(4) * u1.x - u2.x);
+  const C dvdy = hs * (C(-3) * u0.y + C(4) * u1.y - u2.y);
+  const C dwdy = hs * (C(-3) * u0.z + C(4) * u1.z - u2.z);
+  const C dudx = C(0.5) * (uxp.x - uxm.x), dvdx = C(0.5) * (uxp.y - uxm.y), dwdx = C(0.5) * (uxp.z - uxm.z);
+  const C dudz = C(0.5) * (uzp.x - uzm.x), dvdz = C(0.5) * (uzp.y - uzm.y), dwdz = C(0.5) * (uzp.z - uzm.z);
+  // non-equilibrium part: Hermite coefficients -tau S without the collision's
+  // 1/2, i.e. N = -(2/3) tau S in the moment-space form
+  const C k = C(-2.0 / 3.0 * a.tau);
+  NeqTensor<C> N;
+  N.xx = k * dudx; N.yy = k * dvdy; N.zz = k * dwdz;
+  N.xy = k * C(0.5) * (dudy + dvdx);
+  N.xz = k * C(0.5) * (dudz + dwdx);
+  N.yz = k * C(0.5) * (dvdz + dwdy);
+  Macro<C> q;
+  q.rho = C(1); q.dr = C(0);
+  q.ux = u0.x; q.uy = u0.y; q.uz = u0.z;
+  C r[27];
+  rr_post_q27<C, true, FM>(q, N, C(a.fx), C(a.fy), C(a.fz), C(1), C(-1), C(1), r);
+  C dps = C(0), din = C(0);
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if constexpr (L::cy(i) != 0) {
+      if ((L::cy(i) > 0) == top) {
+        dps += h[i];
+        din += r[i];
+      }
+    }
+  });
+  const C d = (dps - din) / (C(1.0 / 6.0) + din);
+  static_for<27>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    h[i] = r[i] + d * (r[i] + C(L::w(i)));
+  });
+}
+
+// FDBC pre-pass: macrovar (macrovar.cu:4-31) of rows 1, 2, LY-2, LY-3 from
+// this step's pulled populations, stored for the wall rows' differences.
+// Thread t -> x = t % LX, slot k = (t / LX) % 4 (rows 1, 2, LY-2, LY-3), z.
+template <int S>
+__global__ __launch_bounds__(kBlock) void fdbc_rows_kernel(FusedArgs a) {
+  using C = typename StoreT<S>::C;
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= 4 * a.lx * a.lz) return;
+  const uint32_t x = t % a.lx, r = t / a.lx, k = r & 3u, z = r >> 2;
+  const uint32_t y = k == 0 ? 1u : (k == 1 ? 2u : (k == 2 ? a.ly - 2 : a.ly - 3));
+  C h[27];
+  gather_lane<S, D3Q27>(a, x, y, z, h);
+  const Moments<C> m = moments_q27<C>(h);
+  const Macro<C> q = macro_of<C, kForceAll>(m, C(a.fx), C(a.fy), C(a.fz));
+  reinterpret_cast<Vel3<C>*>(a.u_wall)[(k * a.lz + z) * a.lx + x] = Vel3<C>{q.ux, q.uy, q.uz};
+}
+
+hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s) {
+  const dim3 grid((4 * a.lx * a.lz + kBlock - 1) / kBlock);
+  switch (storage) {
+    case kF64: hipLaunchKernelGGL(fdbc_rows_kernel<kF64>, grid, dim3(kBlock), 0, s, a); break;
+    case kF32: hipLaunchKernelGGL(fdbc_rows_kernel<kF32>, grid, dim3(kBlock), 0, s, a); break;
+    case kF16: hipLaunchKernelGGL(fdbc_rows_kernel<kF16>, grid, dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // One workgroup's cells of one step: workgroup b of the domain described by a.
 //
 // ROWS: the row length lx is a multiple of 64, so every wavefront covers 64
@@ -267,7 +358,8 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
 // OP: kOpRR the RR collision; kOpBGK the baseline BGK operator (bgk_post) with
 // the BGK solver's velocity u = j + F/2 (D3Q27, no LES, no wall
 // regularisation); kOpRR6 the RR collision and wall regularisation with the
-// fourth- to sixth-order terms (rr_post_q27_o6, D3Q27).
+// fourth- to sixth-order terms (rr_post_q27_o6, D3Q27); kOpRRFD the RR
+// collision with FDBC at the y-wall rows (fdbc_treatment, D3Q27).
 template <int S, class L, bool WALLS, bool LES, int MODE, int FM, bool ROWS, int OP = kOpRR>
 RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   using C = typename StoreT<S>::C;
@@ -335,7 +427,11 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-    if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off);
+    if constexpr (OP == kOpRRFD) {
+      if (y == 0 || y == a.ly - 1) fdbc_treatment<S, FM>(a, h, x, y, z);
+    } else {
+      if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off);
+    }
   }
 
   const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
@@ -753,11 +849,18 @@ static hipError_t launch_op_mode(int mode, const FusedArgs& a, uint32_t grid, hi
 template <int S, int OP>
 static hipError_t launch_op_storage(bool walls, int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
   const bool rows = a.lx % 64u == 0;
-  if (walls)
+  if constexpr (OP == kOpRRFD) {
+    // y walls always
+    (void)walls;
     return rows ? launch_op_mode<S, true, true, OP>(mode, a, grid, s)
                 : launch_op_mode<S, true, false, OP>(mode, a, grid, s);
-  return rows ? launch_op_mode<S, false, true, OP>(mode, a, grid, s)
-              : launch_op_mode<S, false, false, OP>(mode, a, grid, s);
+  } else {
+    if (walls)
+      return rows ? launch_op_mode<S, true, true, OP>(mode, a, grid, s)
+                  : launch_op_mode<S, true, false, OP>(mode, a, grid, s);
+    return rows ? launch_op_mode<S, false, true, OP>(mode, a, grid, s)
+                : launch_op_mode<S, false, false, OP>(mode, a, grid, s);
+  }
 }
 
 template <int OP>
@@ -774,6 +877,7 @@ hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const Fuse
                            hipStream_t s) {
   if (op == kOpBGK) return launch_op<kOpBGK>(storage, walls, mode, a, grid, s);
   if (op == kOpRR6) return launch_op<kOpRR6>(storage, walls, mode, a, grid, s);
+  if (op == kOpRRFD) return launch_op<kOpRRFD>(storage, walls, mode, a, grid, s);
   return hipErrorInvalidValue;
 }
 

@@ -40,6 +40,7 @@ struct rrlbm_ctx {
   PopLayout lay{};        // layout of pop_a / pop_b
   void* u_a = nullptr;  // LES velocity buffers (compute type)
   void* u_b = nullptr;
+  void* u_wall = nullptr;  // FDBC: velocities of rows 1, 2, LY-2, LY-3 (compute type)
   double* partials = nullptr;
   double* diag_out = nullptr;
   double* profiles = nullptr;
@@ -197,6 +198,7 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   set_blocks(a, c->nblocks, c->p.xcd_remap);
   a.ref_f = c->ref.f; a.ref_rho = c->ref.rho; a.ref_ux = c->ref.ux; a.ref_uy = c->ref.uy; a.ref_uz = c->ref.uz;
   a.partials = c->partials;
+  a.u_wall = c->u_wall;
   a.lay = c->lay;
   return a;
 }
@@ -211,7 +213,13 @@ uint32_t fused_grid(const rrlbm_ctx* c, const FusedArgs& a) {
 
 int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
   const bool les = c->p.use_les && mode == kStep;
-  const int op = is_bgk(c->p) ? kOpBGK : (is_rr6(c->p) ? kOpRR6 : kOpRR);
+  int op = is_bgk(c->p) ? kOpBGK : (is_rr6(c->p) ? kOpRR6 : kOpRR);
+  if (c->p.wall_rr == 2) {
+    // FDBC: the velocities of rows 1, 2, LY-2, LY-3 from the same source first
+    hipError_t e = launch_fdbc_rows(c->p.storage, a, c->stream);
+    if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "FDBC row pass launch failed: %s", hipGetErrorString(e));
+    op = kOpRRFD;
+  }
   hipError_t e = op != kOpRR ? launch_fused_op(op, c->p.storage, has_walls(c->p), mode, a, fused_grid(c, a), c->stream)
                              : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a),
                                             c->stream);
@@ -222,7 +230,7 @@ int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
 // Two cells per thread for the plain D3Q27 step only when requested: on
 // MI355X one cell per thread is faster for every storage (higher occupancy).
 bool use_pair(const rrlbm_ctx* c) {
-  if (c->q != 27 || c->p.use_les || is_bgk(c->p) || is_rr6(c->p)) return false;
+  if (c->q != 27 || c->p.use_les || is_bgk(c->p) || is_rr6(c->p) || c->p.wall_rr == 2) return false;
   return c->p.cells_per_thread == 2;
 }
 
@@ -472,6 +480,8 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   c->pop_bytes = c->pop_cap = lay.bytes;
   if ((rc = dalloc(c, &c->pop_a, c->pop_bytes))) return bail(rc);
   if ((rc = dalloc(c, &c->pop_b, c->pop_bytes))) return bail(rc);
+  if (p.wall_rr == 2)
+    if ((rc = dalloc(c, &c->u_wall, 12 * (size_t)p.lx * p.lz * compute_size(p.storage)))) return bail(rc);
   if (p.use_les) {
     if ((rc = dalloc(c, &c->u_a, 3 * n * compute_size(p.storage)))) return bail(rc);
     if ((rc = dalloc(c, &c->u_b, 3 * n * compute_size(p.storage)))) return bail(rc);
@@ -493,8 +503,8 @@ int rrlbm_destroy(rrlbm_ctx* c) {
   if (!c) return RRLBM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->ref.f, c->f_temp, c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz, c->ref.fx, c->ref.fy,
-                  c->ref.fz, c->ref.taut, c->pop_a, c->pop_b, c->u_a, c->u_b, c->partials, c->diag_out,
-                  c->profiles};
+                  c->ref.fz, c->ref.taut, c->pop_a, c->pop_b, c->u_a, c->u_b, c->u_wall, c->partials,
+                  c->diag_out, c->profiles};
   for (void* p : ptrs) dfree(p);
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -718,8 +728,8 @@ int rrlbm_init_equilibrium(rrlbm_ctx* c) {
 
 int rrlbm_fused_pack(rrlbm_ctx* c) {
   NEED_REF(c);
-  if (c->p.wall_rr == 2)
-    return fail(c, RRLBM_E_UNSUPPORTED, "the FDBC wall treatment runs through the drop-in kernels only");
+  if (c->p.wall_rr == 2 && c->p.use_les)
+    return fail(c, RRLBM_E_UNSUPPORTED, "the fused path runs FDBC without LES (the drop-in kernels take both)");
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
   CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->lay, c->p.tau, c->p.use_les, c->p.force[0],
                  c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream, is_bgk(c->p)));
@@ -1008,8 +1018,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   if (trial_steps < 1 || trial_steps > 10000) return fail(c, RRLBM_E_INVALID, "trial_steps must be in [1, 10000]");
   if (c->packed)
     return fail(c, RRLBM_E_STATE, "rrlbm_autotune runs before rrlbm_fused_pack (it overwrites the fused buffers)");
-  if (c->p.wall_rr == 2)
-    return fail(c, RRLBM_E_UNSUPPORTED, "the FDBC wall treatment runs through the drop-in kernels only");
+  if (c->p.wall_rr == 2 && c->p.use_les)
+    return fail(c, RRLBM_E_UNSUPPORTED, "the fused path runs FDBC without LES (the drop-in kernels take both)");
   memset(out, 0, sizeof *out);
   const std::string key = tune_key(c->p);
   {

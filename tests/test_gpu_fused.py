@@ -251,3 +251,31 @@ def test_graph_replays_match_eager_steps(rr, les):
         with pytest.raises(rr.RRLBMError) as ei:
             s.graph_launch(1)
         assert ei.value.status == 3
+
+
+@pytest.mark.parametrize("lx", [12, 64])
+def test_fused_fdbc_matches_reference_order(rr, lx):
+    """FDBC at the y walls (wall_rr = 2) in the fused path: a row pass for the
+    velocities of rows 1, 2, LY-2, LY-3, then the step with the wall rows
+    reconstructed; against the oracle's reference-order step."""
+    g = channel_state(oracle.D3Q27_CUDA, lx, 10, 6, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 1e-6, 0))
+    tau = 0.5 + 3 * 0.002
+    steps = 8
+    got_f, got_m, diag = run_fused(rr, g, steps, storage=0, tau=tau, force=(2e-5, 1e-6, 0), wall_y=1, wall_rr=2)
+    g.step(steps, tau, walls=(0, 1, 0), wall_rr=2)
+    assert maxabs(got_f, g.f) < 1e-12
+    for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-12
+    assert abs(diag["mass_excess"] - g.diag()[4]) < 1e-10
+
+
+def test_fused_fdbc_fp32_fp16_vs_fp64(rr):
+    g = channel_state(oracle.D3Q27_CUDA, 64, 24, 16, u_tau=0.006, visc=0.0016, seed=3, force=(1e-5, 0, 0))
+    tau = 0.5 + 3 * 0.0016
+    res = {}
+    for st in (0, 1, 2):
+        _, m, _ = run_fused(rr, g, 50, storage=st, tau=tau, force=(1e-5, 0, 0), wall_y=1, wall_rr=2)
+        res[st] = m
+    umax = np.abs(res[0][1]).max()
+    assert maxabs(res[1][1], res[0][1]) / umax < 2e-5
+    assert maxabs(res[2][1], res[0][1]) / umax < 2e-2

@@ -185,7 +185,7 @@ def test_fdbc_parameter_checks(rr):
         with pytest.raises(rr.RRLBMError) as ei:
             rr.Solver(**kw)
         assert ei.value.status == 6, bad
-    with rr.Solver(lx=8, ly=8, lz=4, wall_y=1, wall_rr=2) as s:
+    with rr.Solver(lx=8, ly=8, lz=4, wall_y=1, wall_rr=2, use_les=1) as s:
         with pytest.raises(rr.RRLBMError) as ei:
             s.fused_pack()
         assert ei.value.status == 6
