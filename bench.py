@@ -239,7 +239,8 @@ def log(msg):
 
 
 def main_configs(args):
-    """C4 LES channel, C3-grid LES and BGK, C2 cavity, C1 Taylor-Green: one JSON line each."""
+    """C4 LES channel; C3 grid with LES, BGK, sixth-order RR and FDBC; C2 cavity; C1 Taylor-Green:
+    one JSON line each."""
     import rrlbm
     from rrlbm import configs
 
@@ -283,6 +284,15 @@ def main_configs(args):
         log("C3 grid RR order 6 storage %d: %.1f MLUPS" % (st, mlups))
         out.append(line_for("MLUPS, channel RR (orders 4-6 on) Re_tau=180 384x128x192 D3Q27",
                             "C3 grid RR6 %s" % STORAGE_NAME[st], mlups, ms, 2 * 27 * STORAGE_BYTES[st],
+                            STORAGE_NAME[st], args.steps, args.warmup, {"tuning": shape_of(tun)}))
+    # C3 grid with the FDBC wall reconstruction in place of RRBC (a row pass + the step)
+    for st in (1, 2):
+        s, meta = configs.channel(storage=st, wall_rr=2)
+        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        s.close()
+        log("C3 grid FDBC storage %d: %.1f MLUPS" % (st, mlups))
+        out.append(line_for("MLUPS, channel RR + FDBC walls Re_tau=180 384x128x192 D3Q27",
+                            "C3 grid FDBC %s" % STORAGE_NAME[st], mlups, ms, 2 * 27 * STORAGE_BYTES[st],
                             STORAGE_NAME[st], args.steps, args.warmup, {"tuning": shape_of(tun)}))
     # C2: cavity 128^3 Re 1000, D3Q19 and D3Q27
     for q in (19, 27):

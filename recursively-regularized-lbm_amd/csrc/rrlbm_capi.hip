@@ -1055,7 +1055,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   base.group_rows = c->p.pop_group_rows;
   base.pad = c->p.pop_pad_bytes;
   std::vector<Candidate> shapes;
-  const bool pair_ok = c->q == 27 && !c->p.use_les;
+  // two cells per thread only where use_pair() would launch the pair kernel
+  const bool pair_ok = c->q == 27 && !c->p.use_les && !is_bgk(c->p) && !is_rr6(c->p) && c->p.wall_rr != 2;
   for (int cpt : {1, 2})
     for (int remap : {0, 1, 4, 32})
       for (int pers : {0, 4, 8}) {
