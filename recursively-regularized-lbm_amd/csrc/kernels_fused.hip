@@ -137,6 +137,9 @@ RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
       rr_post_q27_o6<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
     else
       rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, C(1), C(-1), C(1), out);
+  } else if constexpr (FM == kForceNone) {
+    const NeqTensor<C> N = neq_q27<C, kForceNone>(m, q, C(0), C(0), C(0));
+    rr_post_q19<C, true>(q, N, C(1), C(1), out);
   } else {
     rr_post_generic<L, C>(h, q, Fx, Fy, Fz, C(1), C(-1), out);
   }
@@ -554,6 +557,13 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
         rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
       else
         rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+    } else if constexpr (FM == kForceNone) {
+      // D3Q19 without force: moment-space form
+      const NeqTensor<C> N = neq_q27<C, kForceNone>(m, q, C(0), C(0), C(0));
+      if constexpr (S == kF16)
+        rr_post_q19<C, false>(q, N, omega, C(a.f16_scale), out);
+      else
+        rr_post_q19<C, true>(q, N, omega, C(1), out);
     } else {
       rr_post_generic<L, C>(h, q, Fx, Fy, Fz, omega, C(1), out);
       if constexpr (S == kF16) {
@@ -789,7 +799,11 @@ static hipError_t launch_storage(int q, bool walls, bool les, int fm, int mode, 
   }
   if (q == 19) {
     if (les) return hipErrorInvalidValue;  // WALE is provided on D3Q27 only
-    // the D3Q19 path is the per-direction formulation with a general force
+    // D3Q19: the moment-space form without force, the per-direction
+    // formulation with one
+    if (fm == kForceNone)
+      return walls ? launch_rows<S, D3Q19, true, false, kForceNone>(mode, a, grid, s)
+                   : launch_rows<S, D3Q19, false, false, kForceNone>(mode, a, grid, s);
     return walls ? launch_rows<S, D3Q19, true, false, kForceAll>(mode, a, grid, s)
                  : launch_rows<S, D3Q19, false, false, kForceAll>(mode, a, grid, s);
   }

@@ -423,6 +423,72 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27_o6(const Macro<C>& q, const NeqTensor<C>&
   });
 }
 
+// D3Q19 without body force, in the same moment-space form.  On the 19
+// velocities every monomial containing c_x c_y c_z vanishes, so the
+// post-collision state is a polynomial in the remaining monomials.  The third
+// order is Malaspinas' pairing (Taylor_Green/D3Q19RR.f90:131-136,230-235): for
+// a linear axis b with quadratic axes a, a' the Hermite pair enters as
+//   (4/3)(H_aab X_ab + H_a'a'b X_a'b) + (2/3)(H_aab X_a'b + H_a'a'b X_ab),
+// so with E_ab = rho u_a^2 u_b + omega R_ab (R_ab = N_aa u_b + 2 u_a N_ab, the
+// recursive third-order non-equilibrium tensor):
+//   K[c_a^2 c_b] = 18 E_ab + 9 E_a'b,   K[c_b] = 3 rho u_b - 9 (E_ab + E_a'b)
+// (on D3Q27 the same terms are 13.5 E_ab and -4.5 (E_ab + E_a'b)).  Outputs as
+// rr_post_q27, in D3Q19 order; gs plays no role without a force.
+template <typename C, bool per_direction_w>
+RRLBM_HD RRLBM_INLINE void rr_post_q19(const Macro<C>& q, const NeqTensor<C>& N, C omega, C scale,
+                                       C (&out)[19]) {
+  const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
+  const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
+  const C wn = omega;
+  // E_ab: a quadratic axis, b linear axis
+  const C Exy = rx * ux * uy + wn * (N.xx * uy + C(2) * ux * N.xy);
+  const C Exz = rx * ux * uz + wn * (N.xx * uz + C(2) * ux * N.xz);
+  const C Eyx = ry * uy * ux + wn * (N.yy * ux + C(2) * uy * N.xy);
+  const C Eyz = ry * uy * uz + wn * (N.yy * uz + C(2) * uy * N.yz);
+  const C Ezx = rz * uz * ux + wn * (N.zz * ux + C(2) * uz * N.xz);
+  const C Ezy = rz * uz * uy + wn * (N.zz * uy + C(2) * uz * N.yz);
+  const C K000 = q.dr - C(1.5) * (rx * ux + ry * uy + rz * uz) - C(1.5) * wn * (N.xx + N.yy + N.zz);
+  const C K100 = C(3) * rx - C(9) * (Eyx + Ezx);
+  const C K010 = C(3) * ry - C(9) * (Exy + Ezy);
+  const C K001 = C(3) * rz - C(9) * (Exz + Eyz);
+  const C K200 = C(4.5) * (rx * ux + wn * N.xx);
+  const C K020 = C(4.5) * (ry * uy + wn * N.yy);
+  const C K002 = C(4.5) * (rz * uz + wn * N.zz);
+  const C K110 = C(9) * (rx * uy + wn * N.xy);
+  const C K101 = C(9) * (rx * uz + wn * N.xz);
+  const C K011 = C(9) * (ry * uz + wn * N.yz);
+  const C K210 = C(18) * Exy + C(9) * Ezy, K012 = C(18) * Ezy + C(9) * Exy;
+  const C K120 = C(18) * Eyx + C(9) * Ezx, K102 = C(18) * Ezx + C(9) * Eyx;
+  const C K201 = C(18) * Exz + C(9) * Eyz, K021 = C(18) * Eyz + C(9) * Exz;
+  // tensor-product evaluation (x, then y, then z) at the 19 velocities
+  const Line3<C> L00 = line3(K000, K100, K200);
+  const Line3<C> L10 = line3(K010, K110, K210);
+  const Line3<C> L01 = line3(K001, K101, K201);
+  const Line3<C> L20 = line3_nok2(K020, K120);
+  const Line3<C> L02 = line3_nok2(K002, K102);
+  auto pick = [](const Line3<C>& l, int c) -> C { return c < 0 ? l.m : (c == 0 ? l.z : l.p); };
+  static_for<19>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int cx = D3Q19::cx(i), cy = D3Q19::cy(i), cz = D3Q19::cz(i);
+    C val;
+    if constexpr (cz == 0) {
+      // (cx, cy, 0): L00 + L10 cy + L20 cy^2
+      val = pick(L00, cx);
+      if constexpr (cy != 0) val += C(cy) * pick(L10, cx) + pick(L20, cx);
+    } else if constexpr (cy == 0) {
+      // (cx, 0, cz): L00 + L01 cz + L02 cz^2
+      val = pick(L00, cx) + C(cz) * pick(L01, cx) + pick(L02, cx);
+    } else {
+      // (0, cy, cz): all terms with c_x = 0
+      val = K000 + C(cy) * K010 + K020 + C(cz) * (K001 + K021) + K002 + C(cy * cz) * K011 + C(cy) * K012;
+    }
+    if constexpr (per_direction_w)
+      out[i] = val * C(D3Q19::w(i));
+    else
+      out[i] = val * scale;
+  });
+}
+
 // Per-direction formulation, following collision.cu:41-118 (D3Q27) and the
 // D3Q19 third-order pairs of D3Q19RR.f90:131-136,221-235,263-281.  h are the
 // shifted populations of the cell; writes out_i = g_i - w_i with

@@ -72,6 +72,16 @@ static void cell(const double* f, double rho, double ux, double uy, double uz, c
     } else {
       rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
     }
+  } else if (fast == 11 || fast == 12) {
+    // D3Q19 moment-space path (no body force); 12: the FP16-storage form
+    const Moments<C> m = moments_generic<L, C>(h);
+    const NeqTensor<C> N = neq_q27<C, kForceNone>(m, q, C(0), C(0), C(0));
+    if (fast == 11) {
+      rr_post_q19<C, true>(q, N, omega, C(1), o);
+    } else {
+      rr_post_q19<C, false>(q, N, omega, C(4096.0), o);
+      for (int i = 0; i < L::Q; ++i) o[i] = C(double(o[i]) * L::w(i) / 4096.0);
+    }
   } else {
     rr_post_generic<L, C>(h, q, C(F[0]), C(F[1]), C(F[2]), omega, gs, o);
   }
@@ -93,7 +103,8 @@ extern "C" {
 // q: 27 or 19; single: 1 = FP32 arithmetic; fast: 0 per-direction path, 1/2/3 moment-space D3Q27
 // path with all force components / force along x only / no force, 4/5/6 the same in the
 // FP16-storage form (values scaled by 4096 / w_i); 7/8/9 the sixth-order path with all force
-// components / x only / none, 10 its FP16-storage form
+// components / x only / none, 10 its FP16-storage form; q = 19: 11 the moment-space D3Q19 path
+// without force, 12 its FP16-storage form
 void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double ux, double uy, double uz,
                 const double* F, double tau_eff, int mode, double* out) {
   if (q == 27) {

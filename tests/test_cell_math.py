@@ -165,3 +165,34 @@ def test_sixth_order_cell_fp32(hc):
             got = hc_cell(hc, 27, 1, 7, f, rho, u, F, 0.53, 0)
             ref = oracle.rr_cell(oracle.D3Q27_CUDA, f, rho, *u, F, 0.53, 0)
             assert np.abs(got - ref).max() < 1e-7
+
+
+@pytest.mark.parametrize("fast", [11, 12])
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("consistent", [True, False])
+def test_d3q19_moment_space_matches_oracle(hc, fast, mode, consistent):
+    """The kernels' D3Q19 polynomial form (rr_post_q19, Malaspinas third-order
+    pairs, no body force) against the oracle's per-direction restatement."""
+    rng = np.random.default_rng(50 + fast + 2 * mode)
+    for _ in range(50):
+        f, rho, u, _ = random_cell(oracle.D3Q19, rng, consistent)
+        F = np.zeros(3)
+        if consistent:
+            c, _, _ = oracle.lattice(oracle.D3Q19)
+            u = (f @ c) / rho
+        tau = rng.uniform(0.5005, 1.5)
+        got = hc_cell(hc, 19, 0, fast, f, rho, u, F, tau, mode)
+        ref = oracle.rr_cell(oracle.D3Q19, f, rho, *u, F, tau, mode)
+        assert np.abs(got - ref).max() < 1e-15
+
+
+def test_d3q19_moment_space_fp32(hc):
+    rng = np.random.default_rng(60)
+    c, _, _ = oracle.lattice(oracle.D3Q19)
+    for _ in range(50):
+        f, rho, u, _ = random_cell(oracle.D3Q19, rng, False)
+        rho = f.sum()
+        u = (f @ c) / rho
+        got = hc_cell(hc, 19, 1, 11, f, rho, u, np.zeros(3), 0.53, 0)
+        ref = oracle.rr_cell(oracle.D3Q19, f, rho, *u, np.zeros(3), 0.53, 0)
+        assert np.abs(got - ref).max() < 1e-7

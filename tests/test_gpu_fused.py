@@ -279,3 +279,16 @@ def test_fused_fdbc_fp32_fp16_vs_fp64(rr):
     umax = np.abs(res[0][1]).max()
     assert maxabs(res[1][1], res[0][1]) / umax < 2e-5
     assert maxabs(res[2][1], res[0][1]) / umax < 2e-2
+
+
+@pytest.mark.parametrize("dims", [(10, 9, 7), (64, 5, 3)])
+def test_fused_d3q19_without_force(rr, dims):
+    """D3Q19 without body force runs the moment-space form (rr_post_q19):
+    periodic box against the oracle's reference-order step, FP64."""
+    g = random_state(oracle.D3Q19, *dims, seed=13, force=(0.0, 0.0, 0.0))
+    tau, steps = 0.55, 12
+    got_f, got_m, _ = run_fused(rr, g, steps, storage=0, tau=tau, force=(0.0, 0.0, 0.0), wall_y=0, wall_rr=0)
+    g.step(steps, tau, walls=(0, 0, 0), wall_rr=0)
+    assert maxabs(got_f, g.f) < 1e-12
+    for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-12
