@@ -21,6 +21,11 @@
 //   --reference        drop-in kernels in the main.cu order instead of fused steps
 //   --ndiag/--nout/--nflowout N   output periods (1000 / 1000 / 100000)
 //   --dir D            output directory (data)   --continue N  restart step
+//   --wall fdbc|rrbc|bb  wall treatment: RRBC (main.cu:74-75, default), FDBC in its
+//                      place (main.cu:76-77, BC.cu:129-309) or plain bounce-back
+//   --rr-order 3|6     RR expansion order (6: the fourth- to sixth-order terms the
+//                      reference keeps commented out in collision.cu / BC.cu)
+//   --bgk              the BGK operator of BGK/collision.cu (plain bounce-back walls)
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -91,7 +96,14 @@ int main(int argc, char** argv) {
     else if (a == "--nflowout") nflowout = atoi(next());
     else if (a == "--dir") dir = next();
     else if (a == "--continue") cont = atoi(next());
-    else {
+    else if (a == "--wall") {
+      const std::string w = next();
+      p.wall_rr = w == "fdbc" ? 2 : (w == "bb" ? 0 : 1);
+    } else if (a == "--rr-order") p.rr_order = atoi(next());
+    else if (a == "--bgk") {
+      p.collision_model = RRLBM_COLLISION_BGK;
+      p.wall_rr = 0;
+    } else {
       fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
     }
