@@ -251,17 +251,17 @@ def main_configs(args):
     for st in (2, 1, 0):
         for lagged in (1, 0):
             s, meta = configs.channel_les(storage=st, les_lagged=lagged)
-            mlups, ms, tun = timed(s, args.steps, args.warmup, graph=True)
+            mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune, graph=not args.no_graph)
             s.close()
             log("C4 storage %d lagged %d: %.1f MLUPS" % (st, lagged, mlups))
             blu = 2 * 27 * STORAGE_BYTES[st] + 6 * (8 if st == 0 else 4) + (0 if lagged else 27 * STORAGE_BYTES[st])
             out.append(line_for("MLUPS, channel LES-WALE Re_tau=180 192x64x96 D3Q27", "C4 LES %s %s" % (
                 STORAGE_NAME[st], "lagged" if lagged else "reference-order"), mlups, ms, blu, STORAGE_NAME[st],
-                args.steps, args.warmup, {"tuning": shape_of(tun), "launch": "graph"}))
+                args.steps, args.warmup, {"tuning": shape_of(tun), "launch": "eager" if args.no_graph else "graph"}))
     # C3 grid with WALE (the DNS grid, LES on): the fused RR+WALE kernel at full size
     for st in (2, 1):
         s, meta = configs.channel(storage=st, les=True)
-        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune)
         s.close()
         log("C3 grid LES storage %d: %.1f MLUPS" % (st, mlups))
         out.append(line_for("MLUPS, channel RR+WALE Re_tau=180 384x128x192 D3Q27", "C3 grid LES %s lagged"
@@ -270,7 +270,7 @@ def main_configs(args):
     # C3 grid under the BGK baseline operator (A/B against the RR step on the same grid)
     for st in (1, 2):
         s, meta = configs.channel_bgk(storage=st)
-        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune)
         s.close()
         log("C3 grid BGK storage %d: %.1f MLUPS" % (st, mlups))
         out.append(line_for("MLUPS, channel BGK Re_tau=180 384x128x192 D3Q27", "C3 grid BGK %s" % STORAGE_NAME[st],
@@ -279,7 +279,7 @@ def main_configs(args):
     # C3 grid with the fourth- to sixth-order RR terms
     for st in (1, 2):
         s, meta = configs.channel(storage=st, rr_order=6)
-        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune)
         s.close()
         log("C3 grid RR order 6 storage %d: %.1f MLUPS" % (st, mlups))
         out.append(line_for("MLUPS, channel RR (orders 4-6 on) Re_tau=180 384x128x192 D3Q27",
@@ -288,7 +288,7 @@ def main_configs(args):
     # C3 grid with the FDBC wall reconstruction in place of RRBC (a row pass + the step)
     for st in (1, 2):
         s, meta = configs.channel(storage=st, wall_rr=2)
-        mlups, ms, tun = timed(s, args.steps, args.warmup)
+        mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune)
         s.close()
         log("C3 grid FDBC storage %d: %.1f MLUPS" % (st, mlups))
         out.append(line_for("MLUPS, channel RR + FDBC walls Re_tau=180 384x128x192 D3Q27",
@@ -298,7 +298,7 @@ def main_configs(args):
     for q in (19, 27):
         for st in (1, 0):
             s, meta = configs.cavity(storage=st, npop=q)
-            mlups, ms, tun = timed(s, args.steps, args.warmup)
+            mlups, ms, tun = timed(s, args.steps, args.warmup, tune=not args.no_tune)
             s.close()
             log("C2 D3Q%d storage %d: %.1f MLUPS" % (q, st, mlups))
             out.append(line_for("MLUPS, lid-driven cavity 128^3 Re=1000", "C2 cavity D3Q%d %s" % (q, STORAGE_NAME[st]),
@@ -309,17 +309,21 @@ def main_configs(args):
     cpu = tg_cpu_baseline()
     for st in (0, 1):
         s, meta = configs.taylor_green_2d(storage=st)
-        mlups, ms, _ = timed(s, 2000, 20)
+        mlups, ms, _ = timed(s, 2000, 20, tune=not args.no_tune)
         # 40 000 cells are launch-bound: the same 2 000 steps as 20 replays of a
         # captured graph of 100 steps
-        s.graph_capture(100)
-        s.graph_launch(1)
-        s.synchronize()
-        t0 = time.perf_counter()
-        gms = s.graph_launch_timed(20)
-        gwall = time.perf_counter() - t0
-        s.close()
-        gmlups = s.n * 2000 / gwall / 1e6
+        if args.no_graph:
+            gms, gmlups = ms * 2000, mlups
+            s.close()
+        else:
+            s.graph_capture(100)
+            s.graph_launch(1)
+            s.synchronize()
+            t0 = time.perf_counter()
+            gms = s.graph_launch_timed(20)
+            gwall = time.perf_counter() - t0
+            s.close()
+            gmlups = s.n * 2000 / gwall / 1e6
         log("C1 storage %d: %.1f MLUPS eager, %.1f MLUPS graph" % (st, mlups, gmlups))
         out.append(line_for("MLUPS, Taylor-Green 100x100x4", "C1 TG %s" % STORAGE_NAME[st], max(mlups, gmlups),
                             min(ms, gms / 2000), 2 * 27 * STORAGE_BYTES[st], STORAGE_NAME[st], 2000, 20,
@@ -432,6 +436,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="library default layout and launch shape (no autotuning)")
+    ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
     ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "configs", "validate", "copy"])
     ap.add_argument("--cavity-steps", type=int, default=20000)
