@@ -358,7 +358,7 @@ hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s) {
 // then wavefront-uniform (scalar registers): each population's source address
 // is a scalar row base plus one of three per-lane x offsets, instead of a
 // per-lane three-term sum.  Otherwise (any lx) every lane computes its own.
-// OP: kOpRR the RR collision; kOpBGK the baseline BGK operator (bgk_post) with
+// OP: kOpRR the RR collision; kOpBGK the baseline BGK operator (bgk_post_q27) with
 // the BGK solver's velocity u = j + F/2 (D3Q27, no LES, no wall
 // regularisation); kOpRR6 the RR collision and wall regularisation with the
 // fourth- to sixth-order terms (rr_post_q27_o6, D3Q27); kOpRRFD the RR
@@ -541,9 +541,9 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       const C inv_tau = C(1) - omega;
       const C fcoef = C(1) - C(0.5) * inv_tau;
       if constexpr (S == kF16)
-        bgk_post<L, C, false>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(a.f16_scale), out);
+        bgk_post_q27<C, false>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(a.f16_scale), out);
       else
-        bgk_post<L, C, true>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(1), out);
+        bgk_post_q27<C, true>(h, q.dr, q.ux, q.uy, q.uz, Fx, Fy, Fz, inv_tau, fcoef, C(1), out);
     } else if constexpr (OP == kOpRR6) {
       static_assert(Q == 27, "the sixth-order terms need D3Q27");
       const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);

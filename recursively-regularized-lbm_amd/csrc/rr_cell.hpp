@@ -606,6 +606,67 @@ RRLBM_HD RRLBM_INLINE void bgk_post(const C (&h)[L::Q], C dr, C ux, C uy, C uz, 
   });
 }
 
+// The same BGK update on D3Q27 in moment space: out_i = (1 - 1/tau) h_i +
+// w_i P(c_i), where P = (1/tau)(dr + Q - 1) + 3 fcoef (c - u).F Q with
+// Q = 1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2, is a polynomial in c of degree <= 2 per
+// axis (c_a^3 = c_a on the lattice): its 17 coefficients are formed once and
+// evaluated at the 27 velocities by the tensor-product stages of rr_post_q27.
+// Outputs as bgk_post (h scaled by S / w_i when per_direction_w is false).
+template <typename C, bool per_direction_w>
+RRLBM_HD RRLBM_INLINE void bgk_post_q27(const C (&h)[27], C dr, C ux, C uy, C uz, C Fx, C Fy, C Fz, C inv_tau,
+                                        C fcoef, C scale, C (&out)[27]) {
+  const C q0 = C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz);
+  const C f3 = C(3) * fcoef;
+  const C px = f3 * Fx, py = f3 * Fy, pz = f3 * Fz;   // phi = 3 fcoef F
+  const C beta = -(px * ux + py * uy + pz * uz);       // -3 fcoef u.F
+  const C t = inv_tau + beta;
+  const C K000 = inv_tau * (dr + q0 - C(1)) + beta * q0;
+  const C K100 = C(3) * ux * t + px * (q0 + C(4.5) * ux * ux);
+  const C K010 = C(3) * uy * t + py * (q0 + C(4.5) * uy * uy);
+  const C K001 = C(3) * uz * t + pz * (q0 + C(4.5) * uz * uz);
+  const C K200 = C(4.5) * ux * ux * t + C(3) * px * ux;
+  const C K020 = C(4.5) * uy * uy * t + C(3) * py * uy;
+  const C K002 = C(4.5) * uz * uz * t + C(3) * pz * uz;
+  const C K110 = C(9) * ux * uy * t + C(3) * (py * ux + px * uy);
+  const C K101 = C(9) * ux * uz * t + C(3) * (pz * ux + px * uz);
+  const C K011 = C(9) * uy * uz * t + C(3) * (pz * uy + py * uz);
+  // c_a^2 c_b: 9/2 phi_b u_a^2 + 9 phi_a u_a u_b
+  const C K210 = C(4.5) * py * ux * ux + C(9) * px * ux * uy;
+  const C K201 = C(4.5) * pz * ux * ux + C(9) * px * ux * uz;
+  const C K120 = C(4.5) * px * uy * uy + C(9) * py * uy * ux;
+  const C K021 = C(4.5) * pz * uy * uy + C(9) * py * uy * uz;
+  const C K102 = C(4.5) * px * uz * uz + C(9) * pz * uz * ux;
+  const C K012 = C(4.5) * py * uz * uz + C(9) * pz * uz * uy;
+  const C K111 = C(9) * (px * uy * uz + py * ux * uz + pz * ux * uy);
+  const Line3<C> L00 = line3(K000, K100, K200);
+  const Line3<C> L10 = line3(K010, K110, K210);
+  const Line3<C> L01 = line3(K001, K101, K201);
+  const Line3<C> L20 = line3_nok2(K020, K120);
+  const Line3<C> L02 = line3_nok2(K002, K102);
+  const Line3<C> L11 = line3_nok2(K011, K111);
+  auto pick = [](const Line3<C>& l, int c) -> C { return c < 0 ? l.m : (c == 0 ? l.z : l.p); };
+  const C keep = C(1) - inv_tau;
+  static_for<3>([&](auto jx) {
+    constexpr int cx = decltype(jx)::value - 1;
+    const Line3<C> M0 = line3(pick(L00, cx), pick(L10, cx), pick(L20, cx));
+    const Line3<C> M1 = line3(pick(L01, cx), pick(L11, cx), K021);
+    const Line3<C> M2 = line3_nok2(pick(L02, cx), K012);
+    static_for<3>([&](auto jy) {
+      constexpr int cy = decltype(jy)::value - 1;
+      const Line3<C> V = line3(pick(M0, cy), pick(M1, cy), pick(M2, cy));
+      static_for<3>([&](auto jz) {
+        constexpr int cz = decltype(jz)::value - 1;
+        constexpr int i = D3Q27::index(cx, cy, cz);
+        const C p = pick(V, cz);
+        if constexpr (per_direction_w)
+          out[i] = keep * h[i] + C(D3Q27::w(i)) * p;
+        else
+          out[i] = (keep * C(1.0 / D3Q27::w(i))) * (h[i] * scale) + scale * p;
+      });
+    });
+  });
+}
+
 // Shifted RR equilibrium of (rho, u): feq_i - w_i (collision.cu:57-69).
 template <class L, typename C>
 RRLBM_HD RRLBM_INLINE void feq_shifted(C rho, C ux, C uy, C uz, C (&out)[L::Q]) {

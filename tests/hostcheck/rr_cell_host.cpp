@@ -115,6 +115,32 @@ void hc_rr_cell(int q, int single, int fast, const double* f, double rho, double
     else cell<D3Q19, double>(f, rho, ux, uy, uz, F, tau_eff, mode, fast, out);
   }
 }
+// BGK update of one D3Q27 cell, shifted populations in and out.  poly: 0 the
+// per-direction bgk_post, 1 the moment-space bgk_post_q27, 2 its FP16-storage
+// form (values scaled by 4096 / w_i, unscaled here).
+void hc_bgk_cell(int single, int poly, const double* h, double dr, double ux, double uy, double uz, const double* F,
+                 double tau, double* out) {
+  auto run = [&](auto zero) {
+    using C = decltype(zero);
+    C hh[27], o[27];
+    const double kS = 4096.0;
+    for (int i = 0; i < 27; ++i) hh[i] = C(poly == 2 ? h[i] * kS / D3Q27::w(i) : h[i]);
+    const C it = C(1.0 / tau), fc = C(1.0 - 0.5 / tau);
+    if (poly == 0)
+      bgk_post<D3Q27, C, true>(hh, C(dr), C(ux), C(uy), C(uz), C(F[0]), C(F[1]), C(F[2]), it, fc, C(1), o);
+    else if (poly == 1)
+      bgk_post_q27<C, true>(hh, C(dr), C(ux), C(uy), C(uz), C(F[0]), C(F[1]), C(F[2]), it, fc, C(1), o);
+    else {
+      // the kernels unscale on load: h = stored * w / S
+      for (int i = 0; i < 27; ++i) hh[i] = C(h[i]);
+      bgk_post_q27<C, false>(hh, C(dr), C(ux), C(uy), C(uz), C(F[0]), C(F[1]), C(F[2]), it, fc, C(kS), o);
+      for (int i = 0; i < 27; ++i) o[i] = C(double(o[i]) * D3Q27::w(i) / kS);
+    }
+    for (int i = 0; i < 27; ++i) out[i] = double(o[i]);
+  };
+  if (single) run(0.0f);
+  else run(0.0);
+}
 void hc_moments(int q, const double* f, double* m) {
   if (q == 27) moments<D3Q27>(f, m);
   else moments<D3Q19>(f, m);

@@ -196,3 +196,32 @@ def test_d3q19_moment_space_fp32(hc):
         got = hc_cell(hc, 19, 1, 11, f, rho, u, np.zeros(3), 0.53, 0)
         ref = oracle.rr_cell(oracle.D3Q19, f, rho, *u, np.zeros(3), 0.53, 0)
         assert np.abs(got - ref).max() < 1e-7
+
+
+def oracle_bgk_cell(hs, dr, u, F, tau):
+    """One cell of the oracle's collision_BGK (BGK/collision.cu:3-33), shifted populations."""
+    g = oracle.Fields(oracle.D3Q27_CUDA, 1, 1, 1)
+    g.f[:, 0] = hs
+    g.rho[:] = dr
+    g.ux[:], g.uy[:], g.uz[:] = u
+    g.set_force(F)
+    g.collision_bgk(tau)
+    return g.f[:, 0].copy()
+
+
+@pytest.mark.parametrize("poly", [0, 1, 2])
+def test_bgk_cell_forms_match_oracle(hc, poly):
+    """The per-direction BGK update and its moment-space form (FP64, and the
+    FP16-storage scaling of the latter) against the oracle's collision_BGK."""
+    rng = np.random.default_rng(70 + poly)
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    for _ in range(50):
+        u = 0.1 * rng.uniform(-1, 1, 3)
+        dr = 0.02 * rng.standard_normal()
+        F = 2e-4 * rng.standard_normal(3)
+        hs = w * (dr + 3 * (c @ u) + 4.5 * (c @ u) ** 2 - 1.5 * u @ u) + 3e-3 * w * rng.standard_normal(27)
+        tau = rng.uniform(0.51, 1.5)
+        out = np.zeros(27)
+        hc.hc_bgk_cell(0, poly, p(hs), ctypes.c_double(dr), ctypes.c_double(u[0]), ctypes.c_double(u[1]),
+                       ctypes.c_double(u[2]), p(F), ctypes.c_double(tau), p(out))
+        assert np.abs(out - oracle_bgk_cell(hs, dr, u, F, tau)).max() < 1e-16
