@@ -1,0 +1,96 @@
+"""Full-size configurations, checked through properties that do not depend on
+the grid size (the oracle is too slow at these sizes): the fused path against
+the drop-in kernels in the reference order at FP64, FP32 / FP16 against FP64
+on the same grid, conservation and bit-identity of the launch forms.
+
+Grids: the channel DNS 384 x 128 x 192 (9.4 M cells), the LES channel
+192 x 64 x 96, the 128^3 cavity, the 64-instance Taylor-Green ensemble
+(1.4 x 10^8 cells)."""
+import numpy as np
+import pytest
+
+from helpers import maxabs
+
+pytestmark = pytest.mark.gpu
+
+
+def fused_vs_reference(rr, make, steps, **kw):
+    s, _ = make(storage=0, **kw)
+    with s:
+        s.step_reference(steps)
+        ref_f = s.download(rr.Field.F)
+        ref_m = s.macroscopic()
+    s, _ = make(storage=0, **kw)
+    with s:
+        s.fused_pack()
+        s.fused_step(steps - 1)
+        s.fused_unpack()
+        f = s.download(rr.Field.F)
+        m = s.macroscopic()
+    return maxabs(f, ref_f), max(maxabs(a, b) for a, b in zip(m, ref_m))
+
+
+def test_channel_dns_fused_equals_reference_order(rr):
+    from rrlbm import configs
+    ef, em = fused_vs_reference(rr, configs.channel, 4)
+    assert ef < 1e-13 and em < 1e-13, (ef, em)
+
+
+def test_channel_les_fused_equals_reference_order(rr):
+    """The LES grid in the reference order (velocity pass + step: exact)."""
+    from rrlbm import configs
+    ef, em = fused_vs_reference(rr, configs.channel_les, 4, les_lagged=0)
+    assert ef < 1e-13 and em < 1e-13, (ef, em)
+
+
+@pytest.mark.parametrize("npop", [19, 27])
+def test_cavity_fused_equals_reference_order(rr, npop):
+    from rrlbm import configs
+    ef, em = fused_vs_reference(rr, configs.cavity, 4, npop=npop)
+    assert ef < 1e-13 and em < 1e-13, (ef, em)
+
+
+def test_channel_dns_storage_precisions(rr):
+    """FP32 / FP16 against FP64 after 100 fused steps of the full channel."""
+    from rrlbm import configs
+    u = {}
+    for st in (0, 1, 2):
+        s, meta = configs.channel(storage=st)
+        with s:
+            s.fused_pack()
+            s.fused_step(99)
+            d = s.fused_diag()
+            s.fused_unpack()
+            u[st] = (s.download(rr.Field.UX), d)
+    umax = np.abs(u[0][0]).max()
+    assert maxabs(u[1][0], u[0][0]) / umax < 1e-5
+    assert maxabs(u[2][0], u[0][0]) / umax < 2e-3
+    for st in (1, 2):
+        assert abs(u[st][1]["u_sum"][0] / u[0][1]["u_sum"][0] - 1) < 1e-5
+
+
+def test_ensemble_conservation_and_launch_forms(rr):
+    """The 64-instance ensemble: batched = per-instance launches bit for bit
+    (FP32), mass and momentum of every periodic box conserved, kinetic energy
+    bounded."""
+    from rrlbm import configs
+    inst = configs.ensemble_instances()
+    res = {}
+    for mode in ("batched", "sequential"):
+        with rr.Batch(inst, storage=1, keep_reference=False) as b:
+            b.init_taylor_green_3d()
+            d0 = [b.diag(k) for k in range(len(inst))]
+            if mode == "batched":
+                b.step(20)
+            else:
+                b.step_sequential(20)
+            res[mode] = [b.diag(k) for k in range(len(inst))]
+    for k, (lx, ly, lz, tau, u0) in enumerate(inst):
+        a, s = res["batched"][k], res["sequential"][k]
+        assert a["kinetic_energy"] == s["kinetic_energy"] and a["mass_excess"] == s["mass_excess"], k
+        n = lx * ly * lz
+        assert abs(a["mass_excess"] - d0[k]["mass_excess"]) < 1e-6 * n ** 0.5 + 1e-9, k
+        assert np.abs(np.array(a["momentum"])).max() < 1e-7 * n * u0, k
+        # uniform initial density: acoustic transients move a little energy
+        # between kinetic and compressive parts over the first steps
+        assert abs(a["kinetic_energy"] / d0[k]["kinetic_energy"] - 1) < 0.05, k
