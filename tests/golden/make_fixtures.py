@@ -16,6 +16,9 @@ Fixtures (plain text, data only):
       i, v, p of the oracle's pure-FP64 run of the same Taylor-Green case
       (for the FP64 GPU comparison; the Fortran output carries REAL*4
       intermediates).
+  cavity_ghia_re1000.txt, cavity_ghia_re400.txt
+      the Ghia et al. centre-line velocities the reference's Cavity solver
+      ships for comparison (Cavity/dataref/Re1000.dat, RE400.dat).
   laminar_60x20x2_uy_profiles.txt
       the reference's committed FP64 fields of the Laminar forced channel
       (Laminar/output/u{x,y,z}{5000,10000,60000}.dat, Fortran unformatted,
@@ -74,6 +77,19 @@ def tg2d_c1_ke(dst):
                header="oracle FP64 kinetic energy, Taylor-Green 100x100x4, u0 = 0.01 cs, Re = 100\ncolumns: step KE")
 
 
+REF_CAVITY = "/root/reference/Cavity/dataref"
+
+
+def cavity_ghia(dst_dir):
+    """The Ghia et al. centre-line data the reference's Cavity solver ships
+    (Cavity/dataref/Re1000.dat, RE400.dat): columns y, u(x = 1/2, y) / U and
+    x, v(x, y = 1/2) / U, lid at y = 1 moving in +x."""
+    for src, re in (("Re1000.dat", 1000), ("RE400.dat", 400)):
+        a = np.loadtxt(os.path.join(REF_CAVITY, src))
+        np.savetxt(os.path.join(dst_dir, "cavity_ghia_re%d.txt" % re), a, fmt="%.5f",
+                   header="Ghia et al. lid-driven cavity, Re = %d (Cavity/dataref/%s)\ncolumns: y u x v" % (re, src))
+
+
 REF_LAMINAR = "/root/reference/Laminar/output"
 
 
@@ -114,6 +130,7 @@ def main():
     oracle_fp64_midline(os.path.join(HERE, "tg_d3q27_L100_it20000_oracle_fp64_midline.txt"))
     tg2d_c1_ke(os.path.join(HERE, "tg2d_c1_ke_oracle.txt"))
     laminar_profiles(os.path.join(HERE, "laminar_60x20x2_uy_profiles.txt"))
+    cavity_ghia(HERE)
 
 
 if __name__ == "__main__":

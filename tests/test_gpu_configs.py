@@ -202,3 +202,45 @@ def test_laminar_channel_against_reference_output(rr):
         umax = np.abs(got[0][k][2]).max()
         for st in (1, 2):
             assert maxabs(got[st][k][2], got[0][k][2]) < 5e-3 * umax, (st, k, maxabs(got[st][k][2], got[0][k][2]))
+
+
+def cavity_centre_lines(rr, npop, re, steps, storage=1, n=128):
+    """The reference's Cavity solver setup (Cavity/var_inc.f90:4,18, para.f90:24-27):
+    128 x 128 x 2, walls on the x and y faces, z periodic, lid y = 1 moving in
+    +x at u0 = 0.1, nu = u0 n / Re, RR regularisation of the wall populations.
+    Returns (y, u(x = 1/2, y) / u0, x, v(x, y = 1/2) / u0) at the cell centres."""
+    lid = 0.1
+    nu = lid * n / re
+    with rr.Solver(lx=n, ly=n, lz=2, npop=npop, storage=storage, tau=3 * nu + 0.5, wall_x=1, wall_y=1, wall_z=0,
+                   wall_rr=1, lid_face=3, lid_u=(lid, 0.0, 0.0)) as s:
+        s.init_rest()
+        s.fused_pack()
+        s.graph_capture(1000)
+        s.graph_launch(steps // 1000)
+        s.synchronize()
+        s.fused_unpack()
+        U = s.download(rr.Field.UX).reshape(2, n, n)[0]
+        V = s.download(rr.Field.UY).reshape(2, n, n)[0]
+    c = n // 2
+    pos = (np.arange(n) + 0.5) / n      # half-way walls at 0 and 1
+    return pos, 0.5 * (U[:, c - 1] + U[:, c]) / lid, pos, 0.5 * (V[c - 1, :] + V[c, :]) / lid
+
+
+@pytest.mark.parametrize("npop", [27, 19])
+@pytest.mark.parametrize("re", [400, 1000])
+def test_cavity_against_ghia(rr, npop, re):
+    """Loose benchmark: the steady centre-line velocities of the quasi-2D
+    cavity against the Ghia et al. data the reference ships
+    (Cavity/dataref/Re1000.dat, RE400.dat), 100 000 steps as Cavity/main.f90:26."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "cavity_ghia_re%d.txt" % re))[1:-1]
+    y, u, x, v = cavity_centre_lines(rr, npop, re, 100000)
+    keep = np.ones(len(ref), dtype=bool)
+    if re == 400:
+        # the tabulated v(x = 0.9063) = -0.23827 breaks the profile's smoothness
+        # (its neighbours -0.44993 at 0.8594 and -0.22847 at 0.9453 with the
+        # near-wall slope of 4.5 put it near -0.40): left out as a table error
+        keep = np.abs(ref[:, 2] - 0.9063) > 1e-4
+    du = np.abs(np.interp(ref[:, 0], y, u) - ref[:, 1]).max()
+    dv = np.abs(np.interp(ref[keep, 2], x, v) - ref[keep, 3]).max()
+    print("cavity D3Q%d Re %d: max |u - u_Ghia| = %.4f, max |v - v_Ghia| = %.4f (of the lid speed)" % (npop, re, du, dv))
+    assert du < 0.03 and dv < 0.03
