@@ -30,6 +30,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <errno.h>
+#include <sys/stat.h>
 #include <sys/time.h>
 
 #include <string>
@@ -116,6 +118,11 @@ int main(int argc, char** argv) {
   printf("Non-dimensional time, %.6e\n", p.ly / u0 / 2);
   printf("Driving force, %.6e\n", p.force[0]);
 
+  // the output directory (statistics.dat, Profiles.txt, the field files)
+  if (mkdir(dir.c_str(), 0755) != 0 && errno != EEXIST) {
+    fprintf(stderr, "cannot create %s: %s\n", dir.c_str(), strerror(errno));
+    return 1;
+  }
   rrlbm_ctx* ctx = nullptr;
   CALL(nullptr, rrlbm_create(&p, &ctx));
   if (cont > 0) {
@@ -157,6 +164,8 @@ int main(int argc, char** argv) {
     const bool want_diag = istep % ndiag == 0, want_prof = istep % nout == 0;
     const bool want_flow = istep % nflowout == 0 || istep == steps, want_f = istep == steps;
     if (want_diag || want_prof || want_flow || want_f) {
+      // finish the queued steps first, so that they count as computation
+      CALL(ctx, rrlbm_synchronize(ctx));
       const double ts = now();
       if (packed) {
         // the fused state holds this step's post-collision populations: finish
@@ -172,8 +181,6 @@ int main(int argc, char** argv) {
         printf("Step %-7d umean = %.17e, vmean = %.17e, wmean = %.17e, rhomean = %14.6e\n", istep, um / u0, vm, wm, rm);
         fflush(stdout);
         std::string path = dir + "/statistics.dat";
-        std::string mk = "mkdir -p '" + dir + "'";
-        if (system(mk.c_str()) != 0) {}
         FILE* fs = fopen(path.c_str(), "a");
         if (!fs) {
           fprintf(stderr, "cannot open %s\n", path.c_str());
