@@ -228,10 +228,19 @@ int rrlbm_init_equilibrium(rrlbm_ctx* ctx);
 
 /* ------------------------------------------------------------ fused path */
 /* Collide the reference-layout state (the next step's collision_RR, using
- * params.force and, with use_les, tau_t from LES_WALE) into the fused buffer. */
+ * params.force -- or the force fields when rrlbm_forcing_gp wrote them last --
+ * and, with use_les, tau_t from LES_WALE) into the fused buffer. */
 int rrlbm_fused_pack(rrlbm_ctx* ctx);
-/* nsteps fused time steps. */
+/* nsteps fused time steps under the uniform params.force. */
 int rrlbm_fused_step(rrlbm_ctx* ctx, int nsteps);
+/* One fused time step whose collision is step istep of main.cu's loop during
+ * its forcing_GP phase (main.cu:33-46, macrovar.cu:76-114): the force is
+ * computed per cell in the kernel (no force fields are read or written); the
+ * step the launch completes keeps the force its collision had.  D3Q27 RR
+ * channel, params.force = (force_in_x, 0, 0).  The loop of main.cu becomes
+ *   forcing_gp(istep0) + fused_pack, then fused_step_gp(istep) while
+ *   istep < npforcing, then fused_step. */
+int rrlbm_fused_step_gp(rrlbm_ctx* ctx, int istep, double rand_phase);
 /* As rrlbm_fused_step, timing the launches with HIP events on the context's
  * stream; *ms receives the elapsed milliseconds. */
 int rrlbm_fused_step_timed(rrlbm_ctx* ctx, int nsteps, double* ms);

@@ -111,6 +111,10 @@ struct FusedArgs {
   double* partials;     // diagnostics pass: per-workgroup FP64 partial sums
   void* u_wall;         // FDBC: velocity of rows 1, 2, LY-2, LY-3 after this step's streaming,
                         // [4][LZ][LX] x (ux, uy, uz) in the compute type
+  int32_t gp;           // forcing_GP body force computed per cell (force_in_x = fx, fy = fz = 0):
+  int32_t gp_on[2];     //   [0] the step the launch completes, [1] the step it collides;
+  int32_t gp_istep[2];  //   off: the uniform force; on: forcing_GP(istep, phase)
+  double gp_phase[2];
   uint32_t lx, ly, lz, n;
   FastDiv div_lx, div_lxly;
   uint32_t walls;       // bit 0: x faces, bit 1: y faces, bit 2: z faces

@@ -167,10 +167,11 @@ RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, const
 // O6: regularise with the fourth- to sixth-order terms
 template <int S, class L, int FM, bool O6 = false>
 RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
-                                            uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off) {
+                                            uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off,
+                                            typename StoreT<S>::C Fx, typename StoreT<S>::C Fy,
+                                            typename StoreT<S>::C Fz) {
   using C = typename StoreT<S>::C;
   const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
-  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
   const bool on[6] = {(a.walls & 1) && x == 0,        (a.walls & 1) && x == a.lx - 1,
                       (a.walls & 2) && y == 0,        (a.walls & 2) && y == a.ly - 1,
                       (a.walls & 4) && z == 0,        (a.walls & 4) && z == a.lz - 1};
@@ -258,6 +259,43 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
     constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
     h[i] = load_pop_b<S>(a.src, a.lay.seg, i, ro[cy + 1][cz + 1] + xb[cx + 1], C(L::w(i)) * inv_scale);
   });
+}
+
+// forcing_GP (macrovar.cu:76-114) at cell (x, y, z) for part k of the launch
+// (0: the step being completed, 1: the step being collided), in the compute
+// type, with the drop-in k_forcing_gp's expressions: a time-periodic force in
+// two bands near the walls; the upper band's x phase is the integer ix / LX
+// (zero) plus the random phase, as the reference writes it.  Off: the
+// uniform force (force_in_x, 0, 0).
+template <typename C>
+RRLBM_INLINE __device__ void gp_force(const FusedArgs& a, int k, uint32_t x, uint32_t y, uint32_t z, C& fx, C& fy,
+                                      C& fz) {
+  const C f0 = C(a.fx);
+  fx = f0; fy = C(0); fz = C(0);
+  if (!a.gp_on[k]) return;
+  const C pi2 = C(6.283185307179586476925);
+  const C LYq = C(a.ly / 4);
+  const int h0 = 2;
+  const C beta9 = C(3), gamma9 = C(2);
+  const C LX = C(a.lx), LY = C(a.ly), LZ = C(a.lz);
+  const C Amp0 = (C(100) * beta9 / LY) * sin(pi2 * C(a.gp_istep[k]) / C(2000));
+  const int iy = (int)y;
+  const C ix = C(x), izc = C(z);
+  if (iy > h0 && C(iy) < LYq + C(h0)) {
+    const C sy = sin(pi2 * C(iy - h0) / LYq), cy = C(1) - cos(pi2 * C(iy - h0) / LYq);
+    const C cz = cos(gamma9 * pi2 * izc / LZ);
+    fx = f0 * (C(1) - Amp0 * LX / beta9 * sy * sin(beta9 * pi2 * ix / LX) * cz);
+    fy = f0 * Amp0 * C(0.5) * LYq * cy * cos(beta9 * pi2 * ix / LX) * cz;
+    fz = f0 * Amp0 * C(0.5) * LZ / gamma9 * sy * cos(beta9 * pi2 * ix / LX) * sin(gamma9 * pi2 * izc / LZ);
+  } else if (C(iy) > C(0.75) * LY - C(h0) && iy < (int)a.ly - h0) {
+    const C yy = -LY + C(iy) + C(h0) + LYq;
+    const C xph = C(0) + C(a.gp_phase[k]);
+    const C sy = sin(pi2 * yy / LYq), cy = C(1) - cos(pi2 * yy / LYq);
+    const C cz = cos(gamma9 * pi2 * izc / LZ);
+    fx = f0 * (C(1) + Amp0 * LX / beta9 * sy * sin(beta9 * pi2 * xph) * cz);
+    fy = -f0 * Amp0 * C(0.5) * LYq * cy * cos(beta9 * pi2 * xph) * cz;
+    fz = -f0 * Amp0 * C(0.5) * LZ / gamma9 * sy * cos(beta9 * pi2 * xph) * sin(gamma9 * pi2 * izc / LZ);
+  }
 }
 
 // FDBC at a y-wall cell (BC.cu:129-309, the drop-in k_fdbc_* in one place):
@@ -427,17 +465,27 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     own_off = cell_off<S>(a.lay, cell);
   }
 
+  // body force of the step this launch completes (F) and of the step it
+  // collides (G): the uniform force, or forcing_GP per cell
+  C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  C Gx = Fx, Gy = Fy, Gz = Fz;
+  if constexpr (FM == kForceAll && OP == kOpRR && Q == 27) {
+    if (a.gp) {
+      gp_force<C>(a, 0, x, y, z, Fx, Fy, Fz);
+      gp_force<C>(a, 1, x, y, z, Gx, Gy, Gz);
+    }
+  }
+
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
     if constexpr (OP == kOpRRFD) {
       if (y == 0 || y == a.ly - 1) fdbc_treatment<S, FM>(a, h, x, y, z);
     } else {
-      if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off);
+      if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off, Fx, Fy, Fz);
     }
   }
 
-  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
   Moments<C> m;
   if constexpr (Q == 27) m = moments_q27<C>(h);
   else m = moments_generic<L, C>(h);
@@ -552,11 +600,11 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
       else
         rr_post_q27_o6<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
     } else if constexpr (Q == 27) {
-      const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+      const NeqTensor<C> N = neq_q27<C, FM>(m, q, Gx, Gy, Gz);
       if constexpr (S == kF16)
-        rr_post_q27<C, false, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(a.f16_scale), out);
+        rr_post_q27<C, false, FM>(q, N, Gx, Gy, Gz, omega, C(1), C(a.f16_scale), out);
       else
-        rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+        rr_post_q27<C, true, FM>(q, N, Gx, Gy, Gz, omega, C(1), C(1), out);
     } else if constexpr (FM == kForceNone) {
       // D3Q19 without force: moment-space form
       const NeqTensor<C> N = neq_q27<C, kForceNone>(m, q, C(0), C(0), C(0));
@@ -646,7 +694,7 @@ RRLBM_INLINE __device__ void fused_pair_body(const FusedArgs& a, uint32_t b) {
       const uint32_t x = cx_[k], y = cy_[k], z = cz_[k];
       const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) ||
                            ((a.walls & 2) && (y == 0 || y == a.ly - 1)) || ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-      if (on_wall) wall_treatment<S, L, FM>(a, h[k], x, y, z, a.src, off);
+      if (on_wall) wall_treatment<S, L, FM>(a, h[k], x, y, z, a.src, off, C(a.fx), C(a.fy), C(a.fz));
     }
     const Moments<C> m = moments_q27<C>(h[k]);
     const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
@@ -829,6 +877,7 @@ static hipError_t launch_pair_force(int fm, const FusedArgs& a, uint32_t grid, h
 }
 
 static int force_mode_of(const FusedArgs& a) {
+  if (a.gp) return kForceAll;  // forcing_GP: all three components, per cell
   return (a.fx == 0.0 && a.fy == 0.0 && a.fz == 0.0) ? kForceNone : (a.fy == 0.0 && a.fz == 0.0) ? kForceX : kForceAll;
 }
 

@@ -498,11 +498,17 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(RefFields r, Grid g, Ini
 // streaming / bounce-back / RRBC, with its rho and u) is collided (the next
 // step's collision_RR with the uniform force of the parameters) and stored in
 // the fused kernel's storage.
+// cell_force: the collision takes the force fields (as after forcing_GP)
+// instead of the uniform force fx, fy, fz.
 template <int S, class L>
 __global__ __launch_bounds__(kBlock) void k_pack(RefFields r, Grid g, void* dst, PopLayout lay, double tau, int use_les,
-                                                 double fx, double fy, double fz, double f16_scale, int bgk) {
+                                                 double fx, double fy, double fz, double f16_scale, int bgk,
+                                                 int cell_force) {
   const uint32_t cell = tid();
   if (cell >= g.n) return;
+  if (cell_force) {
+    fx = r.fx[cell]; fy = r.fy[cell]; fz = r.fz[cell];
+  }
   constexpr int Q = L::Q;
   double h[Q];
   static_for<Q>([&](auto ic) {
@@ -695,8 +701,10 @@ hipError_t ref_init_fields(const RefFields& r, const Grid& g, const InitSpec& sp
   RRLBM_LAUNCH(k_init_fields, g, s, r, g, spec);
 }
 hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, const PopLayout& lay, double tau,
-                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk) {
-#define RRLBM_PACK(S, LT) RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, lay, tau, use_les, fx, fy, fz, f16_scale, bgk)
+                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk,
+                    int cell_force) {
+#define RRLBM_PACK(S, LT) \
+  RRLBM_LAUNCH((k_pack<S, LT>), g, s, r, g, dst, lay, tau, use_les, fx, fy, fz, f16_scale, bgk, cell_force)
   if (q == 27) {
     if (storage == kF64) RRLBM_PACK(kF64, D3Q27);
     if (storage == kF32) RRLBM_PACK(kF32, D3Q27);

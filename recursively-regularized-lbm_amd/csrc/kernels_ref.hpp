@@ -54,8 +54,10 @@ hipError_t ref_init_arrays(int q, const RefFields& r, const Grid& g, double usta
                            hipStream_t s);
 hipError_t ref_init_equilibrium(int q, const RefFields& r, const Grid& g, hipStream_t s, int bgk = 0);
 hipError_t ref_init_fields(const RefFields& r, const Grid& g, const InitSpec& spec, hipStream_t s);
+// cell_force: collide with the force fields instead of the uniform fx, fy, fz
 hipError_t ref_pack(int storage, int q, const RefFields& r, const Grid& g, void* dst, const PopLayout& lay, double tau,
-                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk = 0);
+                    int use_les, double fx, double fy, double fz, double f16_scale, hipStream_t s, int bgk = 0,
+                    int cell_force = 0);
 hipError_t ref_copy_u(int storage, const RefFields& r, const Grid& g, void* u, hipStream_t s);
 hipError_t ref_plane_profiles(const RefFields& r, const Grid& g, int with_taut, double* out, hipStream_t s);
 hipError_t ref_diag(const RefFields& r, const Grid& g, double* partials, hipStream_t s);

@@ -41,6 +41,12 @@ struct rrlbm_ctx {
   void* u_a = nullptr;  // LES velocity buffers (compute type)
   void* u_b = nullptr;
   void* u_wall = nullptr;  // FDBC: velocities of rows 1, 2, LY-2, LY-3 (compute type)
+  // body force of the fused state's last collision, and what the force fields
+  // hold: the uniform force or forcing_GP(istep, phase)
+  struct Forcing {
+    int gp = 0, istep = 0;
+    double phase = 0.0;
+  } force_cur, force_fields;
   double* partials = nullptr;
   double* diag_out = nullptr;
   double* profiles = nullptr;
@@ -199,6 +205,13 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.ref_f = c->ref.f; a.ref_rho = c->ref.rho; a.ref_ux = c->ref.ux; a.ref_uy = c->ref.uy; a.ref_uz = c->ref.uz;
   a.partials = c->partials;
   a.u_wall = c->u_wall;
+  // the step this launch completes collided under force_cur; the collision of
+  // a plain step takes the uniform force (fused_steps overrides part 1)
+  a.gp_on[0] = c->force_cur.gp;
+  a.gp_istep[0] = c->force_cur.istep;
+  a.gp_phase[0] = c->force_cur.phase;
+  a.gp_on[1] = 0;
+  a.gp = a.gp_on[0];
   a.lay = c->lay;
   return a;
 }
@@ -235,7 +248,7 @@ bool use_pair(const rrlbm_ctx* c) {
 }
 
 int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
-  if (use_pair(c)) {
+  if (use_pair(c) && !a.gp) {
     FusedArgs p = a;
     set_blocks(p, (c->n + 2 * kBlock - 1) / (2 * kBlock), c->p.xcd_remap);
     hipError_t e = launch_fused_pair(c->p.storage, has_walls(c->p), p, fused_grid(c, p), c->stream);
@@ -245,9 +258,17 @@ int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
   return launch(c, kStep, a);
 }
 
-int fused_steps(rrlbm_ctx* c, int nsteps) {
+// next: the force of the collided steps (nullptr: the uniform force)
+int fused_steps(rrlbm_ctx* c, int nsteps, const rrlbm_ctx::Forcing* next = nullptr) {
   for (int s = 0; s < nsteps; ++s) {
     FusedArgs a = fused_args(c);
+    if (next && next->gp) {
+      a.gp_on[1] = 1;
+      a.gp_istep[1] = next->istep;
+      a.gp_phase[1] = next->phase;
+      a.gp = 1;
+    }
+    c->force_cur = next ? *next : rrlbm_ctx::Forcing{};
     if (c->p.use_les) {
       if (c->p.les_lagged) {
         a.u_in = c->u_a;
@@ -582,12 +603,14 @@ int rrlbm_init_arrays(rrlbm_ctx* c, double ustar, double ystar) {
 int rrlbm_forcing(rrlbm_ctx* c) {
   NEED_REF(c);
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
+  c->force_fields = rrlbm_ctx::Forcing{};
   return RRLBM_OK;
 }
 
 int rrlbm_forcing_gp(rrlbm_ctx* c, int istep, double rand_phase) {
   NEED_REF(c);
   CK(c, ref_forcing_gp(c->ref, c->grid, c->p.force[0], istep, rand_phase, c->stream));
+  c->force_fields = rrlbm_ctx::Forcing{1, istep, rand_phase};
   return RRLBM_OK;
 }
 
@@ -731,8 +754,12 @@ int rrlbm_fused_pack(rrlbm_ctx* c) {
   if (c->p.wall_rr == 2 && c->p.use_les)
     return fail(c, RRLBM_E_UNSUPPORTED, "the fused path runs FDBC without LES (the drop-in kernels take both)");
   if (c->p.use_les) CK(c, ref_les_wale(c->ref, c->grid, c->p.cw, c->stream));
+  // the first collision takes forcing_GP's fields when they hold it
+  // (main.cu:33-59: forcing_GP, then collision_RR)
+  const int gp = c->force_fields.gp;
   CK(c, ref_pack(c->p.storage, c->q, c->ref, c->grid, c->pop_a, c->lay, c->p.tau, c->p.use_les, c->p.force[0],
-                 c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream, is_bgk(c->p)));
+                 c->p.force[1], c->p.force[2], c->p.f16_scale, c->stream, is_bgk(c->p), gp));
+  c->force_cur = gp ? c->force_fields : rrlbm_ctx::Forcing{};
   if (c->p.use_les) CK(c, ref_copy_u(c->p.storage, c->ref, c->grid, c->u_a, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
   c->packed = true;
@@ -744,6 +771,17 @@ int rrlbm_fused_step(rrlbm_ctx* c, int nsteps) {
   if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before fused steps");
   if (nsteps < 0) return fail(c, RRLBM_E_INVALID, "nsteps must be >= 0");
   return fused_steps(c, nsteps);
+}
+
+int rrlbm_fused_step_gp(rrlbm_ctx* c, int istep, double rand_phase) {
+  NEED(c);
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before fused steps");
+  if (c->q != 27 || is_bgk(c->p) || is_rr6(c->p) || c->p.wall_rr == 2 || c->p.force[1] != 0.0 ||
+      c->p.force[2] != 0.0)
+    return fail(c, RRLBM_E_UNSUPPORTED,
+                "forcing_GP in the fused step: the D3Q27 RR channel with force (force_in_x, 0, 0), RRBC or bounce-back walls");
+  const rrlbm_ctx::Forcing next{1, istep, rand_phase};
+  return fused_steps(c, 1, &next);
 }
 
 int rrlbm_fused_step_timed(rrlbm_ctx* c, int nsteps, double* ms) {
@@ -765,6 +803,8 @@ int rrlbm_fused_step_timed(rrlbm_ctx* c, int nsteps, double* ms) {
 int rrlbm_graph_capture(rrlbm_ctx* c, int nsteps) {
   NEED(c);
   if (!c->packed) return fail(c, RRLBM_E_STATE, "rrlbm_fused_pack must run before capturing fused steps");
+  if (c->force_cur.gp)
+    return fail(c, RRLBM_E_STATE, "the fused state's last collision took forcing_GP: run a plain fused step first");
   if (nsteps < 1 || nsteps > 100000) return fail(c, RRLBM_E_INVALID, "nsteps must be in [1, 100000]");
   CK(c, hipStreamSynchronize(c->stream));
   if (c->exec) (void)hipGraphExecDestroy(c->exec);

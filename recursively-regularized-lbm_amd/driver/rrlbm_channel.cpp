@@ -140,26 +140,23 @@ int main(int argc, char** argv) {
   double t_out = 0.0;
   bool packed = false;
   for (int istep = cont + 1; istep <= steps; ++istep) {
+    // main.cu:33-46: forcing_GP (new phase every 2000 steps) below gp_steps,
+    // the uniform force after
     const bool gp = istep < gp_steps;
-    if (gp) {
-      if (istep % 2000 == 0) phase = next_phase(&rng);
-      if (packed) {
-        CALL(ctx, rrlbm_fused_unpack(ctx));
-        packed = false;
-      }
-      CALL(ctx, rrlbm_forcing_gp(ctx, istep, phase));
+    if (gp && istep % 2000 == 0) phase = next_phase(&rng);
+    if (reference || !packed) {
+      if (gp) CALL(ctx, rrlbm_forcing_gp(ctx, istep, phase));
+      else CALL(ctx, rrlbm_forcing(ctx));
     }
-    if (reference || gp) {
-      if (!gp) CALL(ctx, rrlbm_forcing(ctx));
+    if (reference) {
       CALL(ctx, rrlbm_step_reference(ctx, 1));
+    } else if (!packed) {
+      CALL(ctx, rrlbm_fused_pack(ctx));   // this step's collision
+      packed = true;
+    } else if (gp) {
+      CALL(ctx, rrlbm_fused_step_gp(ctx, istep, phase));   // force computed in the kernel
     } else {
-      if (!packed) {
-        CALL(ctx, rrlbm_forcing(ctx));
-        CALL(ctx, rrlbm_fused_pack(ctx));
-        packed = true;
-      } else {
-        CALL(ctx, rrlbm_fused_step(ctx, 1));
-      }
+      CALL(ctx, rrlbm_fused_step(ctx, 1));
     }
     const bool want_diag = istep % ndiag == 0, want_prof = istep % nout == 0;
     const bool want_flow = istep % nflowout == 0 || istep == steps, want_f = istep == steps;

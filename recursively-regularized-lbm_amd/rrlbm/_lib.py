@@ -112,6 +112,7 @@ SIGNATURES = {
     "rrlbm_init_equilibrium": [_P],
     "rrlbm_fused_pack": [_P],
     "rrlbm_fused_step": [_P, _I],
+    "rrlbm_fused_step_gp": [_P, _I, ctypes.c_double],
     "rrlbm_fused_step_timed": [_P, _I, _D],
     "rrlbm_fused_unpack": [_P],
     "rrlbm_fused_diag": [_P, ctypes.POINTER(Diag)],

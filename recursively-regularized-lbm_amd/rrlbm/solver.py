@@ -226,6 +226,11 @@ class Solver:
     def fused_step(self, nsteps=1):
         self._call("rrlbm_fused_step", int(nsteps))
 
+    def fused_step_gp(self, istep, rand_phase):
+        """One fused step whose collision is step istep under forcing_GP
+        (macrovar.cu:76-114), the force computed in the kernel."""
+        self._call("rrlbm_fused_step_gp", int(istep), float(rand_phase))
+
     def fused_step_timed(self, nsteps):
         ms = ctypes.c_double(0.0)
         self._call("rrlbm_fused_step_timed", int(nsteps), ctypes.byref(ms))

@@ -19,6 +19,9 @@ Fixtures (plain text, data only):
   cavity_ghia_re1000.txt, cavity_ghia_re400.txt
       the Ghia et al. centre-line velocities the reference's Cavity solver
       ships for comparison (Cavity/dataref/Re1000.dat, RE400.dat).
+  re180_spectral_uplus.txt
+      y/h, y+, U+ of the spectral Re_tau = 180 channel profile the reference
+      compares with (RR27V33_code_DNS_LES/Re180.prof).
   laminar_60x20x2_uy_profiles.txt
       the reference's committed FP64 fields of the Laminar forced channel
       (Laminar/output/u{x,y,z}{5000,10000,60000}.dat, Fortran unformatted,
@@ -90,6 +93,18 @@ def cavity_ghia(dst_dir):
                    header="Ghia et al. lid-driven cavity, Re = %d (Cavity/dataref/%s)\ncolumns: y u x v" % (re, src))
 
 
+REF_RE180 = "/root/reference/RR27V33_code_DNS_LES/Re180.prof"
+
+
+def re180_profile(dst):
+    """Columns y/h, y+, U+ of the spectral Re_tau = 180 channel profile the
+    reference compares its runs with (RR27V33_code_DNS_LES/Re180.prof,
+    turbulent.ipynb)."""
+    a = np.loadtxt(REF_RE180)
+    np.savetxt(dst, a[:, :3], fmt="%.7e",
+               header="spectral channel Re_tau = 180 mean profile (RR27V33_code_DNS_LES/Re180.prof)\ncolumns: y/h y+ U+")
+
+
 REF_LAMINAR = "/root/reference/Laminar/output"
 
 
@@ -131,6 +146,7 @@ def main():
     tg2d_c1_ke(os.path.join(HERE, "tg2d_c1_ke_oracle.txt"))
     laminar_profiles(os.path.join(HERE, "laminar_60x20x2_uy_profiles.txt"))
     cavity_ghia(HERE)
+    re180_profile(os.path.join(HERE, "re180_spectral_uplus.txt"))
 
 
 if __name__ == "__main__":

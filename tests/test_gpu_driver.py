@@ -26,7 +26,8 @@ def run_driver(tmp_path, name, *extra):
     return stats, r.stdout
 
 
-@pytest.mark.parametrize("variant", [[], ["--wall", "fdbc"], ["--rr-order", "6"], ["--bgk"], ["--les"]])
+@pytest.mark.parametrize("variant", [[], ["--wall", "fdbc"], ["--rr-order", "6"], ["--bgk"], ["--les"],
+                                     ["--gp-steps", "15"], ["--gp-steps", "15", "--les"]])
 def test_driver_fused_equals_reference_order(tmp_path, variant):
     if not os.path.exists(DRIVER):
         pytest.fail("driver not built: run the build first")

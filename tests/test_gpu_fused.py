@@ -292,3 +292,50 @@ def test_fused_d3q19_without_force(rr, dims):
     assert maxabs(got_f, g.f) < 1e-12
     for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
         assert maxabs(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("les", [False, True])
+def test_fused_forcing_gp_matches_reference_order(rr, les):
+    """forcing_GP computed inside the fused kernel (rrlbm_fused_step_gp)
+    against the drop-in loop of main.cu:33-84 (forcing_GP fields + the
+    reference kernels): GP steps with a phase change, then the uniform force."""
+    lx, ly, lz = 32, 24, 8
+    g = channel_state(oracle.D3Q27_CUDA, lx, ly, lz, u_tau=0.01, visc=0.002, seed=5, force=(2e-5, 0, 0))
+    tau = 0.5 + 3 * 0.002
+    kw = dict(lx=lx, ly=ly, lz=lz, storage=0, tau=tau, force=(2e-5, 0, 0), wall_y=1, wall_rr=1, use_les=int(les),
+              les_lagged=0)
+    schedule = [(istep, 0.31 if istep < 607 else 0.77) for istep in range(601, 613)]   # phase change at 607
+    gp_end = 610
+    with rr.Solver(**kw) as s:
+        upload_state(s, g)
+        for istep, ph in schedule:
+            if istep < gp_end:
+                s.forcing_GP(istep, ph)
+            else:
+                s.forcing()
+            s.step_reference(1)
+        ref_f = s.download(rr.Field.F)
+        ref_m = s.macroscopic()
+    with rr.Solver(**kw) as s:
+        upload_state(s, g)
+        istep, ph = schedule[0]
+        s.forcing_GP(istep, ph)
+        s.fused_pack()
+        for istep, ph in schedule[1:]:
+            if istep < gp_end:
+                s.fused_step_gp(istep, ph)
+            else:
+                s.fused_step(1)
+        s.fused_unpack()
+        f = s.download(rr.Field.F)
+        m = s.macroscopic()
+    assert maxabs(f, ref_f) < 1e-12
+    for a, b in zip(m, ref_m):
+        assert maxabs(a, b) < 1e-12
+    # the force mattered: a run without it differs
+    with rr.Solver(**kw) as s:
+        upload_state(s, g)
+        s.fused_pack()
+        s.fused_step(len(schedule) - 1)
+        s.fused_unpack()
+        assert maxabs(s.macroscopic()[1], ref_m[1]) > 1e-7
