@@ -415,6 +415,73 @@ def main_validate(args):
     print(json.dumps({"workload": "validate", "results": res}))
 
 
+def main_turbulence(args):
+    """Long run of the channel DNS (fused path, 384 x 128 x 192) with the
+    reference's protocol (main.cu:29-84: init_arrays, forcing_GP with a new
+    random phase every 2000 steps below step 50 000, then the uniform force)
+    to a statistically steady state; the mean U+ profile over the last window
+    against the spectral Re_tau = 180 profile the reference compares with
+    (Re180.prof), and the bulk velocity against its U_b+ = 15.709."""
+    import numpy as np
+
+    import rrlbm
+    from rrlbm import configs
+
+    rrlbm.load()
+    spec = np.loadtxt(os.path.join(REPO, "tests", "golden", "re180_spectral_uplus.txt"))
+    s, meta = configs.channel(storage=args.turb_storage)
+    u_tau, nu, ly = meta["u_tau"], meta["nu"], s.ly
+    s.init_arrays(u_tau, nu / u_tau)
+    rng = np.random.default_rng(12345)
+    gp_end, phase = args.turb_gp_steps, 0.0
+    t0 = time.perf_counter()
+    s.forcing_GP(1, phase)
+    s.fused_pack()                       # step 1's collision
+    for istep in range(2, gp_end):
+        if istep % 2000 == 0:
+            phase = float(rng.uniform())
+        s.fused_step_gp(istep, phase)
+        if istep % 10000 == 0:
+            s.synchronize()
+            log("turbulence: forcing_GP step %d, %.0f s" % (istep, time.perf_counter() - t0))
+    done, acc, cnt, every, chunks = max(gp_end - 1, 1), None, 0, 1000, 0
+    while done < args.turb_steps:
+        n = min(every, args.turb_steps - done)
+        s.fused_step(n)
+        done += n
+        chunks += 1
+        if chunks % 50 == 0:
+            s.synchronize()
+            log("turbulence: %d steps, %.0f s" % (done, time.perf_counter() - t0))
+        if done > args.turb_steps - args.turb_window:
+            s.fused_unpack()
+            p = s.output_profile()
+            acc = p.copy() if acc is None else acc + p
+            cnt += 1
+    d = s.fused_diag()
+    s.close()
+    prof = acc / cnt
+    # fold the two halves of the channel; cell centres at y + 1/2 from the wall
+    u = 0.5 * (prof[:, 0] + prof[::-1, 0])[: ly // 2]
+    yw = np.arange(ly // 2) + 0.5
+    yplus = yw * u_tau / nu
+    uplus = u / u_tau
+    ref = np.interp(yplus, spec[:, 1], spec[:, 2])
+    band = (yplus > 5) & (yplus < 150)
+    # friction velocity from the wall gradient of the mean profile (first two cells)
+    dudy = (9 * u[0] - u[1]) / 3.0      # quadratic through u = 0 at the half-way wall
+    res = {"workload": "turbulence", "storage": STORAGE_NAME[args.turb_storage], "steps": args.turb_steps,
+           "gp_steps": gp_end, "window": args.turb_window, "samples": cnt,
+           "seconds": round(time.perf_counter() - t0, 1),
+           "ubulk_plus_window": float(prof[:, 0].mean() / u_tau), "ubulk_plus_end": d["u_sum"][0] / d["ncells"] / u_tau,
+           "ubulk_plus_spectral": 15.709, "u_tau_from_wall_gradient": float(np.sqrt(nu * dudy)), "u_tau_imposed": u_tau,
+           "max_abs_dUplus_5_150": float(np.abs(uplus - ref)[band].max()),
+           "max_rel_dUplus_5_150": float((np.abs(uplus - ref) / ref)[band].max()),
+           "centre_uplus": float(uplus[-1]), "centre_uplus_spectral": float(spec[-1, 2]),
+           "profile": {"yplus": [round(float(v), 3) for v in yplus], "uplus": [round(float(v), 4) for v in uplus]}}
+    print(json.dumps(res))
+
+
 def main_copy():
     """Calibration workload for rocprofv3's FETCH_SIZE / WRITE_SIZE counters: a
     plain device-to-device copy of 1 GiB, five times."""
@@ -438,7 +505,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="library default layout and launch shape (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
-    ap.add_argument("--workload", default="channel", choices=["channel", "ensemble", "configs", "validate", "copy"])
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "ensemble", "configs", "validate", "copy", "turbulence"])
+    ap.add_argument("--turb-steps", type=int, default=1000000)
+    ap.add_argument("--turb-window", type=int, default=300000)
+    ap.add_argument("--turb-gp-steps", type=int, default=50000)
+    ap.add_argument("--turb-storage", type=int, default=1)
     ap.add_argument("--cavity-steps", type=int, default=20000)
     ap.add_argument("--les-steps", type=int, default=50000)
     ap.add_argument("--dns-steps", type=int, default=30000)
@@ -452,6 +524,8 @@ def main():
         return main_copy()
     if args.workload == "configs":
         return main_configs(args)
+    if args.workload == "turbulence":
+        return main_turbulence(args)
     if args.workload == "validate":
         return main_validate(args)
     import rrlbm
