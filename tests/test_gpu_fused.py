@@ -339,3 +339,21 @@ def test_fused_forcing_gp_matches_reference_order(rr, les):
         s.fused_step(len(schedule) - 1)
         s.fused_unpack()
         assert maxabs(s.macroscopic()[1], ref_m[1]) > 1e-7
+
+
+@pytest.mark.parametrize("dims,walls", [((1, 1, 1), (0, 0, 0)), ((1, 6, 1), (0, 0, 0)), ((3, 1, 2), (0, 0, 0)),
+                                        ((1, 4, 1), (0, 1, 0)), ((2, 2, 2), (1, 1, 1)), ((65, 3, 1), (0, 1, 0))])
+@pytest.mark.parametrize("kind", [oracle.D3Q27_CUDA, oracle.D3Q19])
+def test_fused_degenerate_grids(rr, dims, walls, kind):
+    """Edge sizes: single cells and single rows (periodic images of the cell
+    itself), two-cell walled boxes, a row of 65 (one full wavefront plus one
+    cell): the fused FP64 step against the oracle's reference-order step."""
+    force = (0.0, 0.0, 0.0) if kind == oracle.D3Q19 else (2e-5, -1e-5, 3e-6)
+    g = random_state(kind, *dims, seed=3, force=force)
+    wall_rr = 1 if any(walls) else 0
+    got_f, got_m, _ = run_fused(rr, g, 6, storage=0, tau=0.6, force=force, wall_x=walls[0], wall_y=walls[1],
+                                wall_z=walls[2], wall_rr=wall_rr)
+    g.step(6, 0.6, walls=walls, wall_rr=wall_rr)
+    assert maxabs(got_f, g.f) < 1e-12
+    for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
+        assert maxabs(a, b) < 1e-12
