@@ -516,7 +516,7 @@ def main():
     ap.add_argument("--dns-steps", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the solver has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         return main_ensemble(args)

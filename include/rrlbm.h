@@ -46,7 +46,7 @@ typedef enum rrlbm_status {
     RRLBM_E_STATE = 3,       /* call out of order (e.g. fused step before pack) */
     RRLBM_E_NUMERIC = 4,     /* non-finite values detected by a diagnostics call */
     RRLBM_E_IO = 5,          /* file open / read / write failure */
-    RRLBM_E_UNSUPPORTED = 6  /* valid request this build does not provide */
+    RRLBM_E_UNSUPPORTED = 6  /* valid request the library does not provide */
 } rrlbm_status;
 
 /* collision operator: the RR operator of the channel solver (RR27V33_code_DNS_LES/

@@ -1,7 +1,7 @@
 """ctypes wrapper of liboracle.so -- the CPU restatement of the reference.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker.  Arrays are numpy FP64 in the
+CPU-baseline leg of bench.py, as the oracle.  Arrays are numpy FP64 in the
 channel solver's layout (populations [Q, N], idx = (iz*LY + iy)*LX + ix);
 the Taylor-Green functions use the Fortran layout f(ip,i,j,k).
 """

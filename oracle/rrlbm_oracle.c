@@ -1,7 +1,7 @@
 /*
  * rrlbm_oracle.c -- CPU restatement of the reference RR-LBM routines.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the GPU library is
  * compared against.  Only tests/, __graft_entry__.smoke() and the CPU-baseline
  * leg of bench.py may load it; the product path (the HIP library and the
  * rrlbm Python package) never calls it and has no CPU fallback.
