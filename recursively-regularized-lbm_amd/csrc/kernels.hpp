@@ -126,6 +126,7 @@ struct FusedArgs {
   double omega;         // 1 - 1/tau (without LES)
   double lid_x, lid_y, lid_z;
   double f16_scale;     // FP16: stored value = (f_i - w_i) * f16_scale / w_i
+  float f16_unscale;    // FP16: 1 / f16_scale in FP32 (set on the host; no per-wavefront division)
   uint32_t f16_round;   // FP16 store: 0 stochastic, 1 nearest
   uint32_t sr_step;     // step counter of the stochastic-rounding dither
   uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)

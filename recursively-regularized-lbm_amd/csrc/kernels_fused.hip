@@ -39,6 +39,71 @@ RRLBM_INLINE __device__ typename StoreT<S>::C load_pop_b(const void* base, uint6
     return C(v);
 }
 
+// Factor from a stored FP16 value to h_i / w_i (1 / f16_scale); 1 otherwise.
+template <int S>
+RRLBM_INLINE __device__ typename StoreT<S>::C inv_scale_of(const FusedArgs& a) {
+  using C = typename StoreT<S>::C;
+  if constexpr (S == kF16)
+    return C(a.f16_unscale);
+  else
+    return C(1);
+}
+
+// Raw moments of a D3Q27 cell from its FP16-stored values g_i = h_i * S / w_i
+// (S = f16_scale, unscale216 = 1 / (216 S)).  The D3Q27 weights are products
+// of the 1-D weights (1/6, 2/3, 1/6) = (1, 4, 1) / 6, so h_i = g_i * k(cx)
+// k(cy) k(cz) / (216 S) with k(0) = 4, k(+-1) = 1: the three tensor-product
+// stages of moments_q27 run on g with the zero-velocity term of each line sum
+// taken four times (an FMA in place of an add), and the ten moments are
+// scaled once at the end.  The FP16 values enter the first stage as operands
+// of gfx950's mixed-precision FMA (v_fma_mix_f32 converts them in the same
+// instruction), so a population costs one instruction here instead of a
+// conversion, a scaling multiply and its share of the line sums.
+RRLBM_INLINE __device__ Moments<float> moments_q27_f16(const _Float16 (&g)[27], float unscale216) {
+  // an opaque 1.0 keeps fma(g+, 1, g-) from being folded into an add of two
+  // separately converted values
+  float one = 1.0f;
+  asm volatile("" : "+s"(one));
+  float X0[3][3], X1[3][3], X2[3][3];
+  static_for<3>([&](auto jy) {
+    static_for<3>([&](auto jz) {
+      constexpr int cy = decltype(jy)::value - 1, cz = decltype(jz)::value - 1;
+      const _Float16 gm = g[D3Q27::index(-1, cy, cz)];
+      const _Float16 g0 = g[D3Q27::index(0, cy, cz)];
+      const _Float16 gp = g[D3Q27::index(1, cy, cz)];
+      const float s = __builtin_fmaf(float(gp), one, float(gm));
+      X2[cy + 1][cz + 1] = s;
+      X1[cy + 1][cz + 1] = __builtin_fmaf(float(gm), -2.0f, s);  // g+ - g-, exact
+      X0[cy + 1][cz + 1] = __builtin_fmaf(float(g0), 4.0f, s);
+    });
+  });
+  float Y00[3], Y01[3], Y02[3], Y10[3], Y11[3], Y20[3];
+  static_for<3>([&](auto kz) {
+    constexpr int z = decltype(kz)::value;
+    const float s0 = X0[2][z] + X0[0][z];
+    Y02[z] = s0;
+    Y00[z] = __builtin_fmaf(X0[1][z], 4.0f, s0);
+    Y01[z] = X0[2][z] - X0[0][z];
+    Y10[z] = __builtin_fmaf(X1[1][z], 4.0f, X1[2][z] + X1[0][z]);
+    Y11[z] = X1[2][z] - X1[0][z];
+    Y20[z] = __builtin_fmaf(X2[1][z], 4.0f, X2[2][z] + X2[0][z]);
+  });
+  auto k3 = [](const float (&v)[3]) { return __builtin_fmaf(v[1], 4.0f, v[2] + v[0]); };
+  Moments<float> m;
+  const float s00 = Y00[2] + Y00[0];
+  m.dr = __builtin_fmaf(Y00[1], 4.0f, s00) * unscale216;
+  m.jz = (Y00[2] - Y00[0]) * unscale216;
+  m.pzz = s00 * unscale216;
+  m.jx = k3(Y10) * unscale216;
+  m.pxz = (Y10[2] - Y10[0]) * unscale216;
+  m.jy = k3(Y01) * unscale216;
+  m.pyz = (Y01[2] - Y01[0]) * unscale216;
+  m.pxx = k3(Y20) * unscale216;
+  m.pyy = k3(Y02) * unscale216;
+  m.pxy = k3(Y11) * unscale216;
+  return m;
+}
+
 template <int S>
 RRLBM_INLINE __device__ uint32_t cell_off(const PopLayout& l, uint32_t c) {
   return pop_cell_off(l, c, (uint32_t)sizeof(typename StoreT<S>::T));
@@ -59,10 +124,13 @@ RRLBM_INLINE __device__ void store_raw(void* base, uint64_t seg, int i, uint32_t
 // on MI355X).  The result is the upper FP16 neighbour with probability
 // (dropped part) / (FP16 step), so the stored value equals v in expectation.
 // A carry into the exponent gives the next binade's first value, as it should.
+// The instruction writes one half of its destination and keeps the other:
+// `z` carries the register from one conversion to the next, so that no
+// conversion needs a cleared register of its own (only the low half is used).
 typedef _Float16 rr_half2 __attribute__((ext_vector_type(2)));
-RRLBM_INLINE __device__ _Float16 f16_stochastic(float v, uint32_t r) {
-  const rr_half2 z = {_Float16(0), _Float16(0)};
-  return __builtin_amdgcn_cvt_sr_f16_f32(z, v, r, false)[0];
+RRLBM_INLINE __device__ _Float16 f16_stochastic(rr_half2& z, float v, uint32_t r) {
+  z = __builtin_amdgcn_cvt_sr_f16_f32(z, v, r, false);
+  return z[0];
 }
 
 // 32-bit integer mixer ("lowbias32", from C. Wellons' hash-prospector search).
@@ -98,9 +166,10 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
     if (a.f16_round == 0) {
       asm volatile("" : "+v"(off));
       const uint32_t r0 = dither_base(cell, a.sr_step, a.sr_salt);
+      rr_half2 z = {_Float16(0), _Float16(0)};
       static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        store_raw<T>(base, a.lay.seg, i, off, f16_stochastic(out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
+        store_raw<T>(base, a.lay.seg, i, off, f16_stochastic(z, out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
       });
       return;
     }
@@ -171,7 +240,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
                                             typename StoreT<S>::C Fx, typename StoreT<S>::C Fy,
                                             typename StoreT<S>::C Fz) {
   using C = typename StoreT<S>::C;
-  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  const C inv_scale = inv_scale_of<S>(a);
   const bool on[6] = {(a.walls & 1) && x == 0,        (a.walls & 1) && x == a.lx - 1,
                       (a.walls & 2) && y == 0,        (a.walls & 2) && y == a.ly - 1,
                       (a.walls & 4) && z == 0,        (a.walls & 4) && z == a.lz - 1};
@@ -243,7 +312,7 @@ RRLBM_INLINE __device__ void gather_lane(const FusedArgs& a, uint32_t x, uint32_
   using C = typename StoreT<S>::C;
   constexpr uint32_t sz = sizeof(typename StoreT<S>::T);
   const uint32_t row_b = a.lx * sz;
-  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  const C inv_scale = inv_scale_of<S>(a);
   const uint32_t xb[3] = {(x + 1 == a.lx ? 0u : x + 1) * sz, x * sz, (x == 0 ? a.lx - 1 : x - 1) * sz};
   const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
   const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
@@ -433,21 +502,29 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   const uint32_t xs[3] = {x + 1 == a.lx ? 0u : x + 1, x, x == 0 ? a.lx - 1 : x - 1};
   const uint32_t ys[3] = {y + 1 == a.ly ? 0u : y + 1, y, y == 0 ? a.ly - 1 : y - 1};
   const uint32_t zs[3] = {z + 1 == a.lz ? 0u : z + 1, z, z == 0 ? a.lz - 1 : z - 1};
-  const C inv_scale = S == kF16 ? C(1.0 / a.f16_scale) : C(1);
+  const C inv_scale = inv_scale_of<S>(a);
   constexpr uint32_t sz = sizeof(T);
   const uint32_t row_b = a.lx * sz;
   const uint32_t xb[3] = {xs[0] * sz, xs[1] * sz, xs[2] * sz};
   const char* src = reinterpret_cast<const char*>(a.src);
   char* dst = reinterpret_cast<char*>(a.dst);
 
+  // FP16 D3Q27 collisions that need only the moments (not the populations
+  // after the wall treatment) keep the gathered FP16 values and take their
+  // moments with moments_q27_f16; wall cells convert them first.
+  constexpr bool kRawF16 = ROWS && S == kF16 && Q == 27 && OP != kOpBGK && MODE != kFinish;
   C h[Q];
+  [[maybe_unused]] _Float16 g[Q];
   if constexpr (ROWS) {
     // nine scalar source-row bases, then the lane's x offset
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-      h[i] = load_pop_b<S>(src + row_offset(a, row_b, ys[cy + 1], zs[cz + 1]), a.lay.seg, i, xb[cx + 1],
-                           C(L::w(i)) * inv_scale);
+      const char* row = src + row_offset(a, row_b, ys[cy + 1], zs[cz + 1]);
+      if constexpr (kRawF16)
+        g[i] = *reinterpret_cast<const _Float16*>(row + (uint64_t)i * a.lay.seg + xb[cx + 1]);
+      else
+        h[i] = load_pop_b<S>(row, a.lay.seg, i, xb[cx + 1], C(L::w(i)) * inv_scale);
     });
   } else {
     gather_lane<S, L>(a, x, y, z, h);
@@ -476,19 +553,36 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
     }
   }
 
+  bool treated = false;  // wall cell: h holds the populations after the wall treatment
   if constexpr (WALLS) {
     const bool on_wall = ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
                          ((a.walls & 4) && (z == 0 || z == a.lz - 1));
-    if constexpr (OP == kOpRRFD) {
-      if (y == 0 || y == a.ly - 1) fdbc_treatment<S, FM>(a, h, x, y, z);
-    } else {
-      if (on_wall) wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off, Fx, Fy, Fz);
+    treated = OP == kOpRRFD ? (y == 0 || y == a.ly - 1) : on_wall;
+    if (treated) {
+      if constexpr (kRawF16) {
+        static_for<Q>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          h[i] = C(g[i]) * (C(L::w(i)) * inv_scale);
+        });
+      }
+      if constexpr (OP == kOpRRFD)
+        fdbc_treatment<S, FM>(a, h, x, y, z);
+      else
+        wall_treatment<S, L, FM, OP == kOpRR6>(a, h, x, y, z, own_src, own_off, Fx, Fy, Fz);
     }
   }
 
   Moments<C> m;
-  if constexpr (Q == 27) m = moments_q27<C>(h);
-  else m = moments_generic<L, C>(h);
+  if constexpr (kRawF16) {
+    if (treated)
+      m = moments_q27<C>(h);
+    else
+      m = moments_q27_f16(g, float(a.f16_unscale * (1.0f / 216.0f)));
+  } else if constexpr (Q == 27) {
+    m = moments_q27<C>(h);
+  } else {
+    m = moments_generic<L, C>(h);
+  }
   Macro<C> q;
   if constexpr (OP == kOpBGK) {
     q.dr = m.dr;

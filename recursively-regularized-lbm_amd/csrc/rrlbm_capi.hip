@@ -199,6 +199,7 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.omega = 1.0 - 1.0 / c->p.tau;
   a.lid_x = c->p.lid_u[0]; a.lid_y = c->p.lid_u[1]; a.lid_z = c->p.lid_u[2];
   a.f16_scale = c->p.f16_scale;
+  a.f16_unscale = (float)(1.0 / c->p.f16_scale);
   a.f16_round = (uint32_t)c->p.f16_rounding;
   a.sr_step = c->sr_step;
   set_blocks(a, c->nblocks, c->p.xcd_remap);
