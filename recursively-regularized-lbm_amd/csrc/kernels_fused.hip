@@ -50,58 +50,17 @@ RRLBM_INLINE __device__ typename StoreT<S>::C inv_scale_of(const FusedArgs& a) {
 }
 
 // Raw moments of a D3Q27 cell from its FP16-stored values g_i = h_i * S / w_i
-// (S = f16_scale, unscale216 = 1 / (216 S)).  The D3Q27 weights are products
-// of the 1-D weights (1/6, 2/3, 1/6) = (1, 4, 1) / 6, so h_i = g_i * k(cx)
-// k(cy) k(cz) / (216 S) with k(0) = 4, k(+-1) = 1: the three tensor-product
-// stages of moments_q27 run on g with the zero-velocity term of each line sum
-// taken four times (an FMA in place of an add), and the ten moments are
-// scaled once at the end.  The FP16 values enter the first stage as operands
-// of gfx950's mixed-precision FMA (v_fma_mix_f32 converts them in the same
-// instruction), so a population costs one instruction here instead of a
-// conversion, a scaling multiply and its share of the line sums.
+// (moments_q27_wscaled; unscale216 = 1 / (216 S)).  The FP16 values enter the
+// first stage as operands of gfx950's mixed-precision FMA (v_fma_mix_f32
+// converts them in the same instruction), so a population costs one
+// instruction here instead of a conversion, a scaling multiply and its share
+// of the line sums.  The 1.0 is made opaque (a scalar register the compiler
+// cannot see into): fma(g+, 1, g-) would otherwise fold into an add of two
+// separately converted values.
 RRLBM_INLINE __device__ Moments<float> moments_q27_f16(const _Float16 (&g)[27], float unscale216) {
-  // an opaque 1.0 keeps fma(g+, 1, g-) from being folded into an add of two
-  // separately converted values
   float one = 1.0f;
   asm volatile("" : "+s"(one));
-  float X0[3][3], X1[3][3], X2[3][3];
-  static_for<3>([&](auto jy) {
-    static_for<3>([&](auto jz) {
-      constexpr int cy = decltype(jy)::value - 1, cz = decltype(jz)::value - 1;
-      const _Float16 gm = g[D3Q27::index(-1, cy, cz)];
-      const _Float16 g0 = g[D3Q27::index(0, cy, cz)];
-      const _Float16 gp = g[D3Q27::index(1, cy, cz)];
-      const float s = __builtin_fmaf(float(gp), one, float(gm));
-      X2[cy + 1][cz + 1] = s;
-      X1[cy + 1][cz + 1] = __builtin_fmaf(float(gm), -2.0f, s);  // g+ - g-, exact
-      X0[cy + 1][cz + 1] = __builtin_fmaf(float(g0), 4.0f, s);
-    });
-  });
-  float Y00[3], Y01[3], Y02[3], Y10[3], Y11[3], Y20[3];
-  static_for<3>([&](auto kz) {
-    constexpr int z = decltype(kz)::value;
-    const float s0 = X0[2][z] + X0[0][z];
-    Y02[z] = s0;
-    Y00[z] = __builtin_fmaf(X0[1][z], 4.0f, s0);
-    Y01[z] = X0[2][z] - X0[0][z];
-    Y10[z] = __builtin_fmaf(X1[1][z], 4.0f, X1[2][z] + X1[0][z]);
-    Y11[z] = X1[2][z] - X1[0][z];
-    Y20[z] = __builtin_fmaf(X2[1][z], 4.0f, X2[2][z] + X2[0][z]);
-  });
-  auto k3 = [](const float (&v)[3]) { return __builtin_fmaf(v[1], 4.0f, v[2] + v[0]); };
-  Moments<float> m;
-  const float s00 = Y00[2] + Y00[0];
-  m.dr = __builtin_fmaf(Y00[1], 4.0f, s00) * unscale216;
-  m.jz = (Y00[2] - Y00[0]) * unscale216;
-  m.pzz = s00 * unscale216;
-  m.jx = k3(Y10) * unscale216;
-  m.pxz = (Y10[2] - Y10[0]) * unscale216;
-  m.jy = k3(Y01) * unscale216;
-  m.pyz = (Y01[2] - Y01[0]) * unscale216;
-  m.pxx = k3(Y20) * unscale216;
-  m.pyy = k3(Y02) * unscale216;
-  m.pxy = k3(Y11) * unscale216;
-  return m;
+  return moments_q27_wscaled<float, _Float16>(g, one, unscale216);
 }
 
 template <int S>

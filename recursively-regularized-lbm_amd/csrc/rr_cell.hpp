@@ -84,6 +84,56 @@ RRLBM_HD RRLBM_INLINE Moments<C> moments_q27(const C (&h)[27]) {
   return m;
 }
 
+// Raw moments from weight-scaled values g_i = h_i * S / w_i (the FP16
+// storage form; T is the stored type).  The D3Q27 weights are products of the
+// 1-D weights (1/6, 2/3, 1/6) = (1, 4, 1) / 6, so h_i = g_i k(cx) k(cy) k(cz)
+// / (216 S) with k(0) = 4, k(+-1) = 1: the three stages of moments_q27 run on
+// g with the zero-velocity term of each line sum taken four times, and the ten
+// moments take unscale216 = 1 / (216 S) at the end.  `one` is 1; the device
+// code passes it opaque so that fma(g+, one, g-) stays one mixed-precision FMA
+// reading both stored values.
+template <typename C, typename T>
+RRLBM_HD RRLBM_INLINE Moments<C> moments_q27_wscaled(const T (&g)[27], C one, C unscale216) {
+  C X0[3][3], X1[3][3], X2[3][3];
+  static_for<3>([&](auto jy) {
+    static_for<3>([&](auto jz) {
+      constexpr int cy = decltype(jy)::value - 1, cz = decltype(jz)::value - 1;
+      const T gm = g[D3Q27::index(-1, cy, cz)];
+      const T g0 = g[D3Q27::index(0, cy, cz)];
+      const T gp = g[D3Q27::index(1, cy, cz)];
+      const C s = fma(C(gp), one, C(gm));
+      X2[cy + 1][cz + 1] = s;
+      X1[cy + 1][cz + 1] = fma(C(gm), C(-2), s);  // g+ - g-
+      X0[cy + 1][cz + 1] = fma(C(g0), C(4), s);
+    });
+  });
+  C Y00[3], Y01[3], Y02[3], Y10[3], Y11[3], Y20[3];
+  static_for<3>([&](auto kz) {
+    constexpr int z = decltype(kz)::value;
+    const C s0 = X0[2][z] + X0[0][z];
+    Y02[z] = s0;
+    Y00[z] = fma(X0[1][z], C(4), s0);
+    Y01[z] = X0[2][z] - X0[0][z];
+    Y10[z] = fma(X1[1][z], C(4), X1[2][z] + X1[0][z]);
+    Y11[z] = X1[2][z] - X1[0][z];
+    Y20[z] = fma(X2[1][z], C(4), X2[2][z] + X2[0][z]);
+  });
+  auto k3 = [](const C (&v)[3]) { return fma(v[1], C(4), v[2] + v[0]); };
+  Moments<C> m;
+  const C s00 = Y00[2] + Y00[0];
+  m.dr = fma(Y00[1], C(4), s00) * unscale216;
+  m.jz = (Y00[2] - Y00[0]) * unscale216;
+  m.pzz = s00 * unscale216;
+  m.jx = k3(Y10) * unscale216;
+  m.pxz = (Y10[2] - Y10[0]) * unscale216;
+  m.jy = k3(Y01) * unscale216;
+  m.pyz = (Y01[2] - Y01[0]) * unscale216;
+  m.pxx = k3(Y20) * unscale216;
+  m.pyy = k3(Y02) * unscale216;
+  m.pxy = k3(Y11) * unscale216;
+  return m;
+}
+
 // Raw moments for any lattice, direction by direction.
 template <class L, typename C>
 RRLBM_HD RRLBM_INLINE Moments<C> moments_generic(const C (&h)[L::Q]) {

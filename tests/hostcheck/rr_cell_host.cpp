@@ -141,6 +141,20 @@ void hc_bgk_cell(int single, int poly, const double* h, double dr, double ux, do
   if (single) run(0.0f);
   else run(0.0);
 }
+// Raw moments of a D3Q27 cell from weight-scaled values g_i = h_i * scale / w_i
+// (moments_q27_wscaled, the FP16 kernels' moment sum); single: FP32 arithmetic.
+void hc_moments_wscaled(int single, const double* g, double scale, double* m) {
+  auto run = [&](auto zero) {
+    using C = decltype(zero);
+    C gg[27];
+    for (int i = 0; i < 27; ++i) gg[i] = C(g[i]);
+    const Moments<C> r = moments_q27_wscaled<C, C>(gg, C(1), C(1.0 / (216.0 * scale)));
+    const C v[10] = {r.dr, r.jx, r.jy, r.jz, r.pxx, r.pyy, r.pzz, r.pxy, r.pxz, r.pyz};
+    for (int k = 0; k < 10; ++k) m[k] = double(v[k]);
+  };
+  if (single) run(0.0f);
+  else run(0.0);
+}
 void hc_moments(int q, const double* f, double* m) {
   if (q == 27) moments<D3Q27>(f, m);
   else moments<D3Q19>(f, m);

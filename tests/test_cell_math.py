@@ -121,6 +121,28 @@ def test_moments_and_feq(hc, q, kind):
         assert np.abs(out - oracle.feq(kind, rho, *u)).max() < 1e-16
 
 
+@pytest.mark.parametrize("single", [0, 1])
+def test_weight_scaled_moments(hc, single):
+    """moments_q27_wscaled -- the FP16 kernels' moment sum, which reads the
+    stored values g_i = h_i S / w_i and takes the D3Q27 weights as the
+    per-axis factors (1, 4, 1)/6 inside its line sums -- against the raw
+    moments of h_i = g_i w_i / S.  A wrong axis factor, sign or index moves
+    a moment by O(|h|)."""
+    rng = np.random.default_rng(11)
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    S = 4096.0
+    for _ in range(50):
+        # FP16-representable stored values
+        g = (rng.standard_normal(27) * 64).astype(np.float16).astype(np.float64)
+        m = np.zeros(10)
+        hc.hc_moments_wscaled(single, p(g), ctypes.c_double(S), p(m))
+        h = g * w / S
+        ref = np.concatenate([[h.sum()], c.T @ h, (c ** 2).T @ h,
+                              [h @ (c[:, 0] * c[:, 1]), h @ (c[:, 0] * c[:, 2]), h @ (c[:, 1] * c[:, 2])]])
+        scale = np.abs(h).sum()
+        assert np.abs(m - ref).max() <= (2e-7 if single else 1e-15) * scale
+
+
 def test_wale_kernel_formula(hc):
     """The kernels' WALE (sqrt forms of the powers, 0/0 -> 0) against the
     long-double matrix definition used to pin the oracle."""
