@@ -20,8 +20,9 @@
  * exits the process; rrlbm_last_error() gives the message of the last failure
  * of a context (or of rrlbm_create when ctx is NULL, per host thread).  Work is
  * enqueued on the context's own HIP stream; calls that return host data
- * synchronise that stream.  One context per host thread; no global mutable
- * state.  Sizes are counts of doubles unless stated.
+ * synchronise that stream.  One context per host thread; the only state
+ * shared between contexts is the process's autotuning cache (rrlbm_autotune,
+ * mutex-protected).  Sizes are counts of doubles unless stated.
  *
  * Velocity ordering: NPOP = 27 uses the channel solver's order
  * (RR27V33_code_DNS_LES/initial.cu:67-69); NPOP = 19 uses the Taylor-Green
