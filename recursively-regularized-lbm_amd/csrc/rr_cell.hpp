@@ -436,12 +436,20 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27_o6(const Macro<C>& q, const NeqTensor<C>&
   const Axis X = axis(q.ux, fx ? Fx : C(0)), Y = axis(q.uy, fyz ? Fy : C(0)), Z = axis(q.uz, fyz ? Fz : C(0));
   // h(0) = -1, h(+-1) = 2
   auto hval = [](int j) { return j == 1 ? C(-1) : C(2); };
-  const C nxx = C(1.5) * omega * N.xx, nyy = C(1.5) * omega * N.yy, nzz = C(1.5) * omega * N.zz;
-  const C nxy = C(9) * omega * N.xy, nxz = C(9) * omega * N.xz, nyz = C(9) * omega * N.yz;
-  const C grho = gs * q.rho;
+  // The value is linear in (dr, rho, N, gs rho): a uniform output scale (FP16
+  // storage) multiplies those once instead of each of the 27 results.
+  const C sc = per_direction_w ? C(1) : scale;
+  const C on = omega * sc;
+  const C nxx = C(1.5) * on * N.xx, nyy = C(1.5) * on * N.yy, nzz = C(1.5) * on * N.zz;
+  const C nxy = C(9) * on * N.xy, nxz = C(9) * on * N.xz, nyz = C(9) * on * N.yz;
+  const C dr_s = q.dr * sc, rho_s = q.rho * sc;
+  const C grho = gs * rho_s;
   static_for<3>([&](auto jxc) {
     constexpr int jx = decltype(jxc)::value;
     const C ax = nxx * hval(jx), dx = nxy * X.D[jx], ex = nxz * X.D[jx];
+    // force along x only: the y and z factors g vanish (kept out rather than
+    // added as zeros, which the compiler may not drop)
+    const C gxr = grho * X.g[jx];
     static_for<3>([&](auto jyc) {
       constexpr int jy = decltype(jyc)::value;
       const C Exy = X.p[jx] + Y.p[jy] + X.p[jx] * Y.p[jy];
@@ -453,21 +461,24 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27_o6(const Macro<C>& q, const NeqTensor<C>&
       if constexpr (fx) {
         TT = X.T2[jx] * Y.T2[jy];
         TU = X.T2[jx] * Y.T1[jy] + X.T1[jx] * Y.T2[jy];
-        Sxy = X.g[jx] + Y.g[jy];
+        if constexpr (fyz) Sxy = X.g[jx] + Y.g[jy];
       }
       static_for<3>([&](auto jzc) {
         constexpr int jz = decltype(jzc)::value;
         constexpr int i = D3Q27::index(jx - 1, jy - 1, jz - 1);
         const C E = Exy + Z.p[jz] * PP;
-        C val = q.dr + q.rho * E + Ap * Z.P[jz] + Bp * hval(jz) + Cp * Z.D[jz];
+        C val = dr_s + rho_s * E + Ap * Z.P[jz] + Bp * hval(jz) + Cp * Z.D[jz];
         if constexpr (fx) {
           const C Qv = PP * Z.P[jz] - TT * Z.p[jz] - TU * Z.T2[jz];
-          val += grho * (Sxy + Z.g[jz]) * Qv;
+          if constexpr (fyz)
+            val += grho * (Sxy + Z.g[jz]) * Qv;
+          else
+            val += gxr * Qv;
         }
         if constexpr (per_direction_w)
           out[i] = val * C(D3Q27::w(i));
         else
-          out[i] = val * scale;
+          out[i] = val;
       });
     });
   });

@@ -73,6 +73,31 @@ def test_rr6_fused_matches_oracle(rr, name):
     assert maxabs(f, g3.f) > 1e-9
 
 
+@pytest.mark.parametrize("name", ["channel", "channel_rows64"])
+def test_rr6_fused_force_x_matches_oracle(rr, name):
+    """The step kernel for a force along x only (the channel's), which leaves
+    out the y and z force terms of the sixth-order output, against the oracle:
+    one workgroup per block and a persistent grid."""
+    dims, params, walls, lid = CASES[name]
+    force = (3e-5, 0.0, 0.0)
+    tau, steps = 0.53, 10
+    for extra in ({}, {"persistent_wg_per_cu": 2}):
+        g = random_state(oracle.D3Q27_CUDA, *dims, seed=23, amp=3e-3, umax=0.1, force=force)
+        with rr.Solver(lx=dims[0], ly=dims[1], lz=dims[2], storage=0, tau=tau, force=force, rr_order=6,
+                       **params, **extra) as s:
+            upload_state(s, g)
+            s.fused_pack()
+            s.fused_step(steps - 1)
+            s.fused_unpack()
+            f = s.download(rr.Field.F)
+            m = s.macroscopic()
+        with oracle.rr_order(6):
+            g.step(steps, tau, walls=walls, wall_rr=params["wall_rr"], lid_face=lid)
+        assert maxabs(f, g.f) < 1e-12
+        for a, b in zip(m, (g.rho, g.ux, g.uy, g.uz)):
+            assert maxabs(a, b) < 1e-12
+
+
 @pytest.mark.parametrize("lx", [32, 64])
 def test_rr6_fp32_fp16_vs_fp64(rr, lx):
     g = channel_state(oracle.D3Q27_CUDA, lx, 24, 16, u_tau=0.006, visc=0.0016, seed=3, force=(1e-5, 0, 0))
