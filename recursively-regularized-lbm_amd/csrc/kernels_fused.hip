@@ -948,9 +948,9 @@ hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32
 // general force (step, finish and diagnostics passes).
 template <int S, bool W, bool R, int OP>
 static hipError_t launch_op_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
-  if constexpr (OP == kOpRR6) {
+  if constexpr (OP == kOpRR6 || OP == kOpRRFD) {
     // a force along x only (the channel's): the step kernel without the y and
-    // z force terms of the sixth-order output
+    // z force terms of the collision and the wall treatment
     if (mode == kStep && force_mode_of(a) == kForceX) {
       if (grid < a.tiles)
         hipLaunchKernelGGL((fused_kernel_persistent<S, D3Q27, W, false, kForceX, R, OP>), dim3(grid), dim3(kBlock),

@@ -253,15 +253,17 @@ def test_graph_replays_match_eager_steps(rr, les):
         assert ei.value.status == 3
 
 
+@pytest.mark.parametrize("force", [(2e-5, 1e-6, 0), (2e-5, 0, 0)])
 @pytest.mark.parametrize("lx", [12, 64])
-def test_fused_fdbc_matches_reference_order(rr, lx):
+def test_fused_fdbc_matches_reference_order(rr, lx, force):
     """FDBC at the y walls (wall_rr = 2) in the fused path: a row pass for the
     velocities of rows 1, 2, LY-2, LY-3, then the step with the wall rows
-    reconstructed; against the oracle's reference-order step."""
-    g = channel_state(oracle.D3Q27_CUDA, lx, 10, 6, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 1e-6, 0))
+    reconstructed; against the oracle's reference-order step (force along x
+    only: the step kernel without the y and z force terms)."""
+    g = channel_state(oracle.D3Q27_CUDA, lx, 10, 6, u_tau=0.01, visc=0.002, seed=9, force=force)
     tau = 0.5 + 3 * 0.002
     steps = 8
-    got_f, got_m, diag = run_fused(rr, g, steps, storage=0, tau=tau, force=(2e-5, 1e-6, 0), wall_y=1, wall_rr=2)
+    got_f, got_m, diag = run_fused(rr, g, steps, storage=0, tau=tau, force=force, wall_y=1, wall_rr=2)
     g.step(steps, tau, walls=(0, 1, 0), wall_rr=2)
     assert maxabs(got_f, g.f) < 1e-12
     for a, b in zip(got_m, (g.rho, g.ux, g.uy, g.uz)):
