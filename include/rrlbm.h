@@ -270,14 +270,16 @@ typedef struct rrlbm_tuning {
     int32_t xcd_remap;
     int32_t persistent_wg_per_cu;
     int32_t candidates;           /* configurations timed; 0 = taken from this process's cache */
-    double ms_per_step;           /* fused step time of the choice in the trial */
+    double ms_per_step;           /* fused step time of the choice in the confirmation trial */
     double ms_per_step_default;   /* the library default's time in the same trial */
 } rrlbm_tuning;
 /* Times ctx's fused step over candidate population layouts (rows per group,
  * padding) and launch shapes (1 or 2 cells per thread, block order over the
- * XCDs: linear, slabs, runs of 4 or 16 blocks; one workgroup per block or a
- * persistent grid of 4 or 8 per compute unit), keeps the fastest for ctx and
- * caches the choice per
+ * XCDs: linear, slabs, runs of 4 or 32 blocks; one workgroup per block or a
+ * persistent grid of 4 or 8 per compute unit), best of three interleaved
+ * rounds of trial_steps steps each; then times the fastest against the
+ * library default in rounds of 3 * trial_steps and keeps the default unless
+ * the fastest still beats it.  The choice applies to ctx and is cached per
  * configuration (grid, lattice, storage, walls, LES, force components) for
  * later calls in this process.  It steps ctx's own fused buffers, so it runs
  * before rrlbm_fused_pack (RRLBM_E_STATE otherwise).  No reference counterpart:

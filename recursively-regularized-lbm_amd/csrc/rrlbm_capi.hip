@@ -1108,7 +1108,20 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   int b2 = 0;
   rc = time_candidates(c, shapes, trial_steps, &ms2, &b2);
   if (rc) return rc;
-  const Candidate best = shapes[b2];
+  // confirmation: the winner against the library default (its layout resolved)
+  // in trials three times as long; a winner that is not faster there is not
+  // kept, since a short trial can favour a candidate by chance
+  Candidate dflt{0, -1, 1, c->nblocks >= 2048 ? 32 : 0, 0};
+  rc = apply_candidate(c, dflt);
+  if (rc) return rc;
+  dflt.group_rows = c->p.pop_group_rows;
+  dflt.pad = c->p.pop_pad_bytes;
+  const std::vector<Candidate> fin = {shapes[b2], dflt};
+  std::vector<double> ms3;
+  int b3 = 0;
+  rc = time_candidates(c, fin, 3 * trial_steps, &ms3, &b3);
+  if (rc) return rc;
+  const Candidate best = fin[b3];
   rc = apply_candidate(c, best);
   if (rc) return rc;
   if (c->pop_cap > c->pop_bytes) {  // shrink to the chosen layout
@@ -1121,9 +1134,9 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   out->cells_per_thread = best.cpt;
   out->xcd_remap = best.remap;
   out->persistent_wg_per_cu = best.pers;
-  out->candidates = (int32_t)(lays.size() + shapes.size());
-  out->ms_per_step = ms2[b2];
-  out->ms_per_step_default = ms1[0];
+  out->candidates = (int32_t)(lays.size() + shapes.size() + fin.size());
+  out->ms_per_step = ms3[b3];
+  out->ms_per_step_default = ms3[1];
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = *out;
