@@ -249,9 +249,14 @@ int rrlbm_fused_step_timed(rrlbm_ctx* ctx, int nsteps, double* ms);
  * the reference launches every kernel from the host each step (main.cu:31-117).
  * Replays must start from the buffer the capture started from: with an odd
  * nsteps every second replay is refused (RRLBM_E_STATE; capture again).  With
- * FP16 storage every replay repeats the captured steps' rounding dither. */
+ * FP16 stochastic rounding the captured steps read their step number from a
+ * device counter of the context that each replay advances, so replays give
+ * the state of as many eager steps, dither included.  A change of population
+ * layout (rrlbm_autotune) drops the captured graph. */
 int rrlbm_graph_capture(rrlbm_ctx* ctx, int nsteps);
-/* Replay the captured graph `times` times (times * nsteps steps). */
+/* Replay the captured graph `times` times (times * nsteps steps) under the
+ * uniform force; RRLBM_E_STATE without a packed state, or when the state's
+ * last collision took forcing_GP (run rrlbm_fused_step once first). */
 int rrlbm_graph_launch(rrlbm_ctx* ctx, int times);
 /* As rrlbm_graph_launch, timed with HIP events on the context's stream (ms). */
 int rrlbm_graph_launch_timed(rrlbm_ctx* ctx, int times, double* ms);
@@ -332,8 +337,8 @@ int rrlbm_batch_step(rrlbm_batch* b, int nsteps);
 /* nsteps, one launch per instance per step, in order on one stream. */
 int rrlbm_batch_step_sequential(rrlbm_batch* b, int nsteps);
 /* Capture nsteps batched launches into a HIP graph (replaces a previous one).
- * With FP16 storage every replay repeats the rounding dither of the captured
- * steps (the step counter is a launch argument fixed at capture). */
+ * With FP16 storage the captured steps read their step number from a device
+ * counter that each replay advances: replays equal as many eager steps. */
 int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps);
 /* Launch the captured graph `times` times (times * nsteps steps). */
 int rrlbm_batch_graph_launch(rrlbm_batch* b, int times);

@@ -7,6 +7,17 @@
                        the committed D3Q27 output was produced), run in _ref/
                        to produce D3Q19 output files.  Nothing from the
                        reference is copied into the repository.
+  _ref/cavity/      -- the reference's lid-driven cavity solver (Cavity/*.f90,
+                       128x128x2, D3Q27 RR, Re 1000, lid x = LX moving in +y)
+                       compiled unmodified from its sources and run until its
+                       first field dump (step 5000, Cavity/main.f90:37),
+                       which tests/golden/make_fixtures.py turns into a
+                       fixture.  Its declarations are type-consistent only
+                       with 8-byte default REAL: collision.f90:9 and
+                       streaming.f90:11 declare the REAL functions feqRR,
+                       feqRR_4 `real*8`, and collision.f90 passes real*8
+                       arguments to REAL dummies; so -fdefault-real-8.  -O0:
+                       the -O2 build of this amdflang crashes at start-up.
 """
 import os
 import shutil
@@ -47,7 +58,62 @@ def build_reference_tg19(run=True):
     return REF_DIR
 
 
+CAVITY_SOURCES = ["var_inc.f90", "para.f90", "RR.f90", "collision.f90", "initial.f90", "streaming.f90", "main.f90"]
+CAVITY_DUMP_STEP = 5000
+CAVITY_CELLS = 128 * 128 * 2
+
+
+def build_reference_cavity(run=True, timeout_s=5400):
+    """Compile Cavity/*.f90 into _ref/cavity/ and (optionally) run it there
+    until its step-5000 dump (32re1000/{rho,ux,uy,uz}5000.dat) is complete;
+    the program itself runs 100 000 steps, so it is stopped then.
+
+    Returns the dump directory, or None when the reference or amdflang is absent."""
+    import time
+
+    src_dir = os.path.join(REFERENCE, "Cavity")
+    if not (os.path.isdir(src_dir) and os.path.exists(AMDFLANG)):
+        return None
+    work = os.path.join(REF_DIR, "cavity")
+    os.makedirs(work, exist_ok=True)
+    exe = os.path.join(work, "cavity")
+    if not os.path.exists(exe):
+        subprocess.run([AMDFLANG, "-O0", "-fdefault-real-8"] + [os.path.join(src_dir, f) for f in CAVITY_SOURCES]
+                       + ["-o", exe], check=True, cwd=work)
+    dump = os.path.join(work, "32re1000")
+    names = [os.path.join(dump, "%s%d.dat" % (q, CAVITY_DUMP_STEP)) for q in ("rho", "ux", "uy", "uz")]
+    whole = 8 * CAVITY_CELLS + 8  # one Fortran record: two 4-byte markers around the data
+
+    def complete():
+        return all(os.path.exists(n) and os.path.getsize(n) == whole for n in names)
+
+    if not run or complete():
+        return dump
+    for n in names:
+        if os.path.exists(n):
+            os.remove(n)
+    proc = subprocess.Popen([exe], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    t0 = time.time()
+    try:
+        while not complete():
+            if proc.poll() is not None:
+                raise RuntimeError("the reference Cavity program ended (status %s) before step %d"
+                                   % (proc.returncode, CAVITY_DUMP_STEP))
+            if time.time() - t0 > timeout_s:
+                raise RuntimeError("the reference Cavity program did not reach step %d in %d s"
+                                   % (CAVITY_DUMP_STEP, timeout_s))
+            time.sleep(5)
+        time.sleep(2)  # the last record is written; let the program close the file
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+    return dump
+
+
 if __name__ == "__main__":
     build_oracle()
     if "--ref" in sys.argv:
         print("reference build:", build_reference_tg19())
+    if "--cavity" in sys.argv:
+        print("reference cavity dump:", build_reference_cavity())

@@ -453,14 +453,17 @@ void orc_macrovar(int kind, int lx, int ly, int lz, const double* f, const doubl
 
 /* RRBC, BC.cu:4-126: RR regularisation of the wall-incoming populations,
  * out_i = feq + fneq - Fbar feq, with (rho, u) from the current populations.
- * Generalised to several walled faces in the order x-,x+,y-,y+,z-,z+
- * (Cavity/streaming.f90:91-292): at a cell on several faces, each face first
- * re-imposes its bounce-back values (from fpost, the post-collision
- * populations, plus the lid term) and then regularises its populations with
- * moments recomputed from the cell's current populations.  For the channel
- * (one face per cell) this is exactly the reference RRBC.  rho/u arrays are
- * read for the first face (as the reference reads the macrovar output) and
- * recomputed for later faces. */
+ * Generalised to several walled faces as the reference's Cavity solver does
+ * it (Cavity/streaming.f90:89-292): face by face in the order x-,x+,y-,y+,
+ * z-,z+, each face sets its incoming populations to their bounce-back values
+ * (from fpost, the post-collision populations, plus the lid term) and then
+ * regularises them with moments recomputed from the cell's current
+ * populations.  While an earlier face is treated, the populations entering
+ * through a later face still hold the values the periodic streaming brought
+ * (from the cell across the domain), as in the Fortran, where that face's
+ * bounce-back has not run yet: a cell on several faces first takes those
+ * values back from fpost.  For the channel (one face per cell) this is
+ * exactly the reference RRBC, which reads rho/u from the macrovar output. */
 void orc_rrbc(int kind, int lx, int ly, int lz, double* f, const double* fpost, const double* fx, const double* fy,
               const double* fz, const double* rho, const double* ux, const double* uy, const double* uz,
               int wall_x, int wall_y, int wall_z, int lid_face, const double* lid_u) {
@@ -473,11 +476,26 @@ void orc_rrbc(int kind, int lx, int ly, int lz, double* f, const double* fpost, 
         for (int iy = 0; iy < ly; ++iy)
             for (int ix = 0; ix < lx; ++ix) {
                 long idx = cidx(ix, iy, iz, lx, ly);
-                int nfaces = 0;
+                int nactive = 0;
+                for (int face = 0; face < 6; ++face)
+                    nactive += walled[face / 2] && on_face(face, ix, iy, iz, lx, ly, lz);
+                if (nactive > 1) {
+                    /* undo the bounce-back: periodic streaming from the upwind cell */
+                    for (int ip = 0; ip < L.q; ++ip) {
+                        int entering = 0;
+                        for (int face = 0; face < 6; ++face)
+                            entering |= walled[face / 2] && on_face(face, ix, iy, iz, lx, ly, lz) &&
+                                        enters_through(&L, ip, face);
+                        if (!entering) continue;
+                        int sx = (ix - L.cx[ip] + lx) % lx, sy = (iy - L.cy[ip] + ly) % ly,
+                            sz = (iz - L.cz[ip] + lz) % lz;
+                        f[ip * N + idx] = fpost[ip * N + cidx(sx, sy, sz, lx, ly)];
+                    }
+                }
                 for (int face = 0; face < 6; ++face) {
                     if (!walled[face / 2] || !on_face(face, ix, iy, iz, lx, ly, lz)) continue;
                     double fc[27], out[27];
-                    if (nfaces > 0 && fpost) {
+                    if (nactive > 1) {
                         for (int ip = 0; ip < L.q; ++ip) {
                             if (!enters_through(&L, ip, face)) continue;
                             double v = fpost[L.opp[ip] * N + idx];
@@ -488,7 +506,7 @@ void orc_rrbc(int kind, int lx, int ly, int lz, double* f, const double* fpost, 
                     }
                     for (int ip = 0; ip < L.q; ++ip) fc[ip] = f[ip * N + idx];
                     double r, u, v, w;
-                    if (nfaces == 0) {
+                    if (nactive == 1) {
                         r = rho[idx]; u = ux[idx]; v = uy[idx]; w = uz[idx];
                     } else {
                         double jx = 0, jy = 0, jz = 0;
@@ -501,7 +519,6 @@ void orc_rrbc(int kind, int lx, int ly, int lz, double* f, const double* fpost, 
                     rr_cell(&L, fc, r, u, v, w, fx[idx], fy[idx], fz[idx], 1.0, 1, out);
                     for (int ip = 0; ip < L.q; ++ip)
                         if (enters_through(&L, ip, face)) f[ip * N + idx] = out[ip];
-                    ++nfaces;
                 }
             }
 }

@@ -129,6 +129,7 @@ struct FusedArgs {
   float f16_unscale;    // FP16: 1 / f16_scale in FP32 (set on the host; no per-wavefront division)
   uint32_t f16_round;   // FP16 store: 0 stochastic, 1 nearest
   uint32_t sr_step;     // step counter of the stochastic-rounding dither
+  const uint32_t* sr_base;  // graph replays: device counter added to sr_step (advanced by each replay), or null
   uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)
   uint32_t nblocks;     // workgroups covering the domain
   uint32_t blocks_per_xcd;
@@ -150,9 +151,13 @@ hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const Fuse
 hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
-// work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list
+// work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list;
+// the dither step is step (+ *step_base when step_base is not null)
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
-                          uint32_t parity, uint32_t step, hipStream_t s);
+                          uint32_t parity, uint32_t step, const uint32_t* step_base, hipStream_t s);
+// Device step counter of captured FP16 steps: *p = v, and *p += inc (one lane each).
+hipError_t launch_counter_set(uint32_t* p, uint32_t v, hipStream_t s);
+hipError_t launch_counter_add(uint32_t* p, uint32_t inc, hipStream_t s);
 // Dither offset of batch instance k (the FP16 stochastic rounding of instance
 // k then differs from that of the other instances at the same cell index).
 inline uint32_t instance_salt(uint32_t k) { return k * 0x68E31DA5u; }

@@ -124,7 +124,10 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
   if constexpr (S == kF16) {
     if (a.f16_round == 0) {
       asm volatile("" : "+v"(off));
-      const uint32_t r0 = dither_base(cell, a.sr_step, a.sr_salt);
+      // captured steps take the step count from the device counter that each
+      // graph replay advances, so replays do not repeat the dither
+      const uint32_t step = a.sr_base ? *a.sr_base + a.sr_step : a.sr_step;
+      const uint32_t r0 = dither_base(cell, step, a.sr_salt);
       rr_half2 z = {_Float16(0), _Float16(0)};
       static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -203,7 +206,12 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
   const bool on[6] = {(a.walls & 1) && x == 0,        (a.walls & 1) && x == a.lx - 1,
                       (a.walls & 2) && y == 0,        (a.walls & 2) && y == a.ly - 1,
                       (a.walls & 4) && z == 0,        (a.walls & 4) && z == a.lz - 1};
-  // half-way bounce-back, faces in the order x-,x+,y-,y+,z-,z+ (last face wins)
+  // Face by face in the order x-,x+,y-,y+,z-,z+ (Cavity/streaming.f90:89-292):
+  // the face's incoming populations take their half-way bounce-back value,
+  // then (wall_rr) the RR regularisation replaces them, with moments of the
+  // cell's current populations.  Populations entering through a later face
+  // still hold the periodically pulled value while an earlier face is
+  // treated, as in the reference, whose bounce-back of that face runs later.
   static_for<6>([&](auto fc) {
     constexpr int face = decltype(fc)::value;
     if (!on[face]) return;
@@ -211,22 +219,7 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
       constexpr int i = decltype(ic)::value;
       if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
     });
-  });
-  if (!a.wall_rr) return;
-  // RR regularisation, face by face; later faces first re-impose their own
-  // bounce-back values (Cavity/streaming.f90:91-292).
-  bool done = false;
-  static_for<6>([&](auto fc) {
-    constexpr int face = decltype(fc)::value;
-    if (!on[face]) return;
-    if (done) {
-      static_for<L::Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
-      });
-    }
-    regularise_face<L, C, face, FM, O6>(h, Fx, Fy, Fz);
-    done = true;
+    if (a.wall_rr) regularise_face<L, C, face, FM, O6>(h, Fx, Fy, Fz);
   });
 }
 
@@ -783,9 +776,10 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
 // the FP16 stochastic-rounding dither).
 template <int S>
 RRLBM_INLINE __device__ void batched_tile(const FusedArgs* __restrict__ table, uint2 w, uint32_t parity,
-                                         uint32_t step) {
+                                         uint32_t step, const uint32_t* step_base) {
   FusedArgs a = table[w.x];
   a.sr_step = step;
+  a.sr_base = step_base;
   if (parity) {
     const void* s = a.src;
     a.src = a.dst;
@@ -800,32 +794,54 @@ RRLBM_INLINE __device__ void batched_tile(const FusedArgs* __restrict__ table, u
 template <int S, bool PERSIST>
 __global__ __launch_bounds__(kBlock) void batched_kernel(const FusedArgs* __restrict__ table,
                                                          const uint2* __restrict__ work, uint32_t ntiles,
-                                                         uint32_t parity, uint32_t step) {
+                                                         uint32_t parity, uint32_t step, const uint32_t* step_base) {
   if constexpr (PERSIST) {
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) batched_tile<S>(table, work[t], parity, step);
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) batched_tile<S>(table, work[t], parity, step, step_base);
   } else {
-    if (blockIdx.x < ntiles) batched_tile<S>(table, work[blockIdx.x], parity, step);
+    if (blockIdx.x < ntiles) batched_tile<S>(table, work[blockIdx.x], parity, step, step_base);
   }
 }
 
 template <int S>
 static hipError_t launch_batched_s(const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
-                                   uint32_t parity, uint32_t step, hipStream_t s) {
+                                   uint32_t parity, uint32_t step, const uint32_t* step_base, hipStream_t s) {
   if (grid < ntiles)
-    hipLaunchKernelGGL((batched_kernel<S, true>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step);
+    hipLaunchKernelGGL((batched_kernel<S, true>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step,
+                       step_base);
   else
-    hipLaunchKernelGGL((batched_kernel<S, false>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step);
+    hipLaunchKernelGGL((batched_kernel<S, false>), dim3(grid), dim3(kBlock), 0, s, table, work, ntiles, parity, step,
+                       step_base);
   return hipGetLastError();
 }
 
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
-                          uint32_t parity, uint32_t step, hipStream_t s) {
+                          uint32_t parity, uint32_t step, const uint32_t* step_base, hipStream_t s) {
   switch (storage) {
-    case kF64: return launch_batched_s<kF64>(table, work, ntiles, grid, parity, step, s);
-    case kF32: return launch_batched_s<kF32>(table, work, ntiles, grid, parity, step, s);
-    case kF16: return launch_batched_s<kF16>(table, work, ntiles, grid, parity, step, s);
+    case kF64: return launch_batched_s<kF64>(table, work, ntiles, grid, parity, step, step_base, s);
+    case kF32: return launch_batched_s<kF32>(table, work, ntiles, grid, parity, step, step_base, s);
+    case kF16: return launch_batched_s<kF16>(table, work, ntiles, grid, parity, step, step_base, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// The step counter of captured FP16 steps: set before a series of graph
+// replays (outside the graph) and advanced at the end of each replay (a
+// kernel node: the graph holds no memset node).  One lane writes.
+__global__ void counter_set_kernel(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+__global__ void counter_add_kernel(uint32_t* p, uint32_t inc) {
+  if (threadIdx.x == 0) *p = *p + inc;
+}
+
+hipError_t launch_counter_set(uint32_t* p, uint32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(counter_set_kernel, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_counter_add(uint32_t* p, uint32_t inc, hipStream_t s) {
+  hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(64), 0, s, p, inc);
+  return hipGetLastError();
 }
 
 // Deterministic final sum of the per-workgroup diagnostics: workgroup k sums
@@ -859,6 +875,9 @@ hipError_t launch_diag_finalize(const double* partials, uint32_t nparts, double*
 
 template <int S, class L, bool W, bool LES, int FM, bool R>
 static hipError_t launch_mode(int mode, const FusedArgs& a, uint32_t grid, hipStream_t s) {
+  // only the step kernel strides over the blocks: a smaller grid for another
+  // pass would leave blocks out
+  if (mode != kStep && grid < a.tiles) return hipErrorInvalidConfiguration;
   if (mode == kStep && grid < a.tiles) {
     hipLaunchKernelGGL((fused_kernel_persistent<S, L, W, LES, FM, R>), dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
@@ -961,6 +980,7 @@ static hipError_t launch_op_mode(int mode, const FusedArgs& a, uint32_t grid, hi
       return hipGetLastError();
     }
   }
+  if (mode != kStep && grid < a.tiles) return hipErrorInvalidConfiguration;
   if (mode == kStep && grid < a.tiles) {
     hipLaunchKernelGGL((fused_kernel_persistent<S, D3Q27, W, false, kForceAll, R, OP>), dim3(grid), dim3(kBlock), 0,
                        s, a);

@@ -159,10 +159,16 @@ __global__ __launch_bounds__(kBlock) void k_macrovar(RefFields r, Grid g, int wa
 }
 
 // RRBC (BC.cu:4-126): at wall cells replace the wall-incoming populations by
-// feq + fneq - Fbar feq.  Several faces (cavity edges) are handled in the
-// order x-,x+,y-,y+,z-,z+; a later face first re-imposes its bounce-back values
-// from the post-collision field fpost and recomputes rho, u from the cell's
-// populations (Cavity/streaming.f90:91-292).
+// feq + fneq - Fbar feq.  Several faces (cavity edges) are treated as the
+// reference's Cavity solver does (Cavity/streaming.f90:89-292): face by face
+// in the order x-,x+,y-,y+,z-,z+, the face's incoming populations take their
+// bounce-back values (from the post-collision field fpost, plus the lid term)
+// and are regularised with rho, u of the cell's current populations, while
+// populations entering through a later face still hold the values the
+// periodic streaming brought.  BB has already written every face's
+// bounce-back values, so such a cell first takes the streamed values back
+// from fpost at the upwind cell.  A cell on one face reads rho, u from the
+// macrovar output, as the reference RRBC does.
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_rrbc(RefFields r, const double* fpost, Grid g) {
   const uint32_t cell = tid();
@@ -170,18 +176,34 @@ __global__ __launch_bounds__(kBlock) void k_rrbc(RefFields r, const double* fpos
   const Cell3 c = coords(cell, g);
   if (!on_any_wall(c, g)) return;
   constexpr int Q = L::Q;
+  int nactive = 0;
+  static_for<6>([&](auto fc) { nactive += on_face(decltype(fc)::value, c, g) ? 1 : 0; });
   double h[Q];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     h[i] = r.f[(size_t)i * g.n + cell] - L::w(i);
   });
+  if (nactive > 1) {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      bool entering = false;
+      static_for<6>([&](auto fc) {
+        constexpr int face = decltype(fc)::value;
+        if constexpr (enters_face(face, L::cx(i), L::cy(i), L::cz(i))) entering = entering || on_face(face, c, g);
+      });
+      if (entering) {
+        const uint32_t x = (c.x + g.lx - L::cx(i)) % g.lx, y = (c.y + g.ly - L::cy(i)) % g.ly,
+                       z = (c.z + g.lz - L::cz(i)) % g.lz;
+        h[i] = fpost[(size_t)i * g.n + lin(x, y, z, g)] - L::w(i);
+      }
+    });
+  }
   const double Fx = r.fx[cell], Fy = r.fy[cell], Fz = r.fz[cell];
-  bool done = false;
   static_for<6>([&](auto fc) {
     constexpr int face = decltype(fc)::value;
     if (!on_face(face, c, g)) return;
     Macro<double> q;
-    if (!done) {
+    if (nactive == 1) {
       q.rho = r.rho[cell]; q.dr = q.rho - 1.0;
       q.ux = r.ux[cell]; q.uy = r.uy[cell]; q.uz = r.uz[cell];
     } else {
@@ -214,7 +236,6 @@ __global__ __launch_bounds__(kBlock) void k_rrbc(RefFields r, const double* fpos
       constexpr int i = decltype(ic)::value;
       if constexpr (enters_face(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = out[i];
     });
-    done = true;
   });
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;

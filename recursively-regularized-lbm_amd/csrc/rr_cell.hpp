@@ -752,26 +752,36 @@ RRLBM_HD RRLBM_INLINE void feq_shifted(C rho, C ux, C uy, C uz, C (&out)[L::Q]) 
   });
 }
 
-// WALE eddy relaxation time tau_t = nu_t/cs^2 from the velocity-gradient
-// tensor (RR27V33_code_DNS_LES/LES.cu:79-97).  The fractional powers use
-// x^1.5 = x sqrt(x), x^2.5 = x^2 sqrt(x), x^1.25 = x sqrt(sqrt(x)); a vanishing
-// gradient (0/0 in the reference) gives tau_t = 0.
+// WALE eddy relaxation time (Nicoud & Ducros 1999, as used by
+// RR27V33_code_DNS_LES/LES.cu:79-97 with filter width 1):
+//   tau_t = nu_t / cs^2 = 3 Cw^2 (Sd:Sd)^(3/2) / ((S:S)^(5/2) + (Sd:Sd)^(5/4)),
+// S the symmetric part of the velocity gradient g (g_ij = du_i/dx_j) and Sd
+// the traceless symmetric part of g.g.  Written with the rotation vector w of
+// g (its antisymmetric part is the cross-product matrix [w]x, and
+// [w]x^2 = w w^T - |w|^2 I): sym(g.g) = S.S + w w^T - |w|^2 I, so
+// Sd = M - (tr M / 3) I with M = S.S + w w^T, and S:S = tr(S.S).  The powers
+// use x^1.5 = x sqrt(x), x^2.5 = x^2 sqrt(x), x^1.25 = x sqrt(sqrt(x)); a
+// vanishing gradient (0/0 in the reference) gives tau_t = 0.
 template <typename C>
 RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz, C dwdx, C dwdy, C dwdz, C cw) {
-  C sijsq = (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx) + (dvdz + dwdy) * (dvdz + dwdy);
-  sijsq = C(0.5) * sijsq + dudx * dudx + dvdy * dvdy + dwdz * dwdz;
-  const C s11d = (C(2) * dudx * dudx - dvdy * dvdy - dwdz * dwdz + dudy * dvdx + dudz * dwdx - C(2) * dvdz * dwdy) *
-                 C(1.0 / 3.0);
-  const C s12d = C(0.5) * (dudx * (dudy + dvdx) + dudy * dvdy + dvdx * dvdy + dvdz * dwdx + dudz * dwdy);
-  const C s13d = C(0.5) * (dudx * (dudz + dwdx) + dudy * dvdz + dvdx * dwdy + dudz * dwdz + dwdx * dwdz);
-  const C s22d = (C(2) * dvdy * dvdy - dudx * dudx - dwdz * dwdz + dudy * dvdx + dvdz * dwdy - C(2) * dudz * dwdx) *
-                 C(1.0 / 3.0);
-  const C s23d = C(0.5) * (dvdy * (dvdz + dwdy) + dudz * dvdx + dudy * dwdx + dvdz * dwdz + dwdy * dwdz);
-  const C s33d = (C(2) * dwdz * dwdz - dudx * dudx - dvdy * dvdy + dudz * dwdx + dvdz * dwdy - C(2) * dudy * dvdx) *
-                 C(1.0 / 3.0);
-  const C sd = s11d * s11d + s22d * s22d + s33d * s33d + C(2) * (s12d * s12d + s13d * s13d + s23d * s23d);
-  const C rs = sqrt(sijsq), rd = sqrt(sd);
-  const C den = sijsq * sijsq * rs + sd * sqrt(rd);
+  const C h = C(0.5);
+  // S (diagonal a, off-diagonal b) and w
+  const C a1 = dudx, a2 = dvdy, a3 = dwdz;
+  const C b12 = h * (dudy + dvdx), b13 = h * (dudz + dwdx), b23 = h * (dvdz + dwdy);
+  const C w1 = h * (dwdy - dvdz), w2 = h * (dudz - dwdx), w3 = h * (dvdx - dudy);
+  // S.S
+  const C q12 = b12 * b12, q13 = b13 * b13, q23 = b23 * b23;
+  const C p11 = a1 * a1 + q12 + q13, p22 = a2 * a2 + q12 + q23, p33 = a3 * a3 + q13 + q23;
+  const C ss = p11 + p22 + p33;  // S:S
+  const C m12 = b12 * (a1 + a2) + b13 * b23 + w1 * w2;
+  const C m13 = b13 * (a1 + a3) + b12 * b23 + w1 * w3;
+  const C m23 = b23 * (a2 + a3) + b12 * b13 + w2 * w3;
+  const C m11 = p11 + w1 * w1, m22 = p22 + w2 * w2, m33 = p33 + w3 * w3;
+  const C t = (m11 + m22 + m33) * C(1.0 / 3.0);
+  const C d1 = m11 - t, d2 = m22 - t, d3 = m33 - t;
+  const C sd = d1 * d1 + d2 * d2 + d3 * d3 + C(2) * (m12 * m12 + m13 * m13 + m23 * m23);  // Sd:Sd
+  const C rs = sqrt(ss), rd = sqrt(sd);
+  const C den = ss * ss * rs + sd * sqrt(rd);
   const C num = C(3) * cw * cw * sd * rd;
   return den > C(0) ? num * recip(den) : C(0);
 }

@@ -54,9 +54,12 @@ struct rrlbm_ctx {
   int cus = 256;  // compute units of the device
   bool packed = false;
   uint32_t sr_step = 0;  // fused steps taken since creation (stochastic-rounding dither)
+  uint32_t* d_step = nullptr;    // device step counter read by captured FP16 steps
+  bool capture_dev_step = false;  // set while capturing FP16 stochastic-rounding steps
   hipGraph_t graph = nullptr;  // captured fused steps (rrlbm_graph_capture)
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
+  bool graph_dev_step = false;  // the captured steps read d_step (advanced by every replay)
   void* graph_pop_a = nullptr;  // buffer the captured sequence starts from
   bool ref_ok = true;  // reference-layout fields allocated
   size_t bytes = 0;
@@ -202,6 +205,7 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   a.f16_unscale = (float)(1.0 / c->p.f16_scale);
   a.f16_round = (uint32_t)c->p.f16_rounding;
   a.sr_step = c->sr_step;
+  a.sr_base = c->capture_dev_step ? c->d_step : nullptr;
   set_blocks(a, c->nblocks, c->p.xcd_remap);
   a.ref_f = c->ref.f; a.ref_rho = c->ref.rho; a.ref_ux = c->ref.ux; a.ref_uy = c->ref.uy; a.ref_uz = c->ref.uz;
   a.partials = c->partials;
@@ -217,12 +221,18 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   return a;
 }
 
-// Workgroups of a launch over a.tiles blocks: one per block, or a persistent
-// grid of persistent_wg_per_cu workgroups per compute unit striding over them.
+// Workgroups of a step launch over a.tiles blocks: one per block, or a
+// persistent grid of persistent_wg_per_cu workgroups per compute unit striding
+// over them.  Only the step kernels stride; the macro, finish and diagnostics
+// passes always launch one workgroup per block (launch_grid).
 uint32_t fused_grid(const rrlbm_ctx* c, const FusedArgs& a) {
   const int k = c->p.persistent_wg_per_cu;
   if (k <= 0) return a.tiles;
   return std::max(1u, std::min<uint32_t>(a.tiles, (uint32_t)k * (uint32_t)c->cus));
+}
+
+uint32_t launch_grid(const rrlbm_ctx* c, int mode, const FusedArgs& a) {
+  return mode == kStep ? fused_grid(c, a) : a.tiles;
 }
 
 int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
@@ -234,9 +244,9 @@ int launch(rrlbm_ctx* c, int mode, const FusedArgs& a) {
     if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "FDBC row pass launch failed: %s", hipGetErrorString(e));
     op = kOpRRFD;
   }
-  hipError_t e = op != kOpRR ? launch_fused_op(op, c->p.storage, has_walls(c->p), mode, a, fused_grid(c, a), c->stream)
-                             : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, fused_grid(c, a),
-                                            c->stream);
+  const uint32_t grid = launch_grid(c, mode, a);
+  hipError_t e = op != kOpRR ? launch_fused_op(op, c->p.storage, has_walls(c->p), mode, a, grid, c->stream)
+                             : launch_fused(c->p.storage, c->q, has_walls(c->p), les, mode, a, grid, c->stream);
   if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "fused kernel launch failed: %s", hipGetErrorString(e));
   return RRLBM_OK;
 }
@@ -291,6 +301,22 @@ int fused_steps(rrlbm_ctx* c, int nsteps, const rrlbm_ctx::Forcing* next = nullp
     if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
   }
   return RRLBM_OK;
+}
+
+// Forget a captured graph: its kernel arguments hold the population buffers
+// and the layout of the time of capture.
+void drop_graph(rrlbm_ctx* c) {
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+  c->graph_pop_a = nullptr;
+  c->graph_dev_step = false;
+}
+
+bool f16_dither(const rrlbm_params& p) {
+  return p.storage == RRLBM_STORE_F16 && p.f16_rounding == RRLBM_F16_STOCHASTIC;
 }
 
 int finalize_diag(rrlbm_ctx* c, uint32_t nparts, rrlbm_diag* out) {
@@ -510,6 +536,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   }
   if ((rc = dalloc(c, (void**)&c->partials, (size_t)c->nblocks * kDiagWidth * sizeof(double)))) return bail(rc);
   if ((rc = dalloc(c, (void**)&c->diag_out, kDiagWidth * sizeof(double)))) return bail(rc);
+  if ((rc = dalloc(c, (void**)&c->d_step, 256))) return bail(rc);
   if ((rc = dalloc(c, (void**)&c->profiles, (size_t)p.ly * 10 * sizeof(double)))) return bail(rc);
   // rho = 1 and the uniform force, as initialize() leaves them
   e = ref_forcing(c->ref, c->grid, p.force[0], p.force[1], p.force[2], c->stream);
@@ -526,7 +553,7 @@ int rrlbm_destroy(rrlbm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->ref.f, c->f_temp, c->ref.rho, c->ref.ux, c->ref.uy, c->ref.uz, c->ref.fx, c->ref.fy,
                   c->ref.fz, c->ref.taut, c->pop_a, c->pop_b, c->u_a, c->u_b, c->u_wall, c->partials,
-                  c->diag_out, c->profiles};
+                  c->diag_out, c->profiles, c->d_step};
   for (void* p : ptrs) dfree(p);
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -579,6 +606,9 @@ int rrlbm_upload(rrlbm_ctx* c, rrlbm_field f, const double* host, size_t count) 
   if (!host || count != want) return fail(c, RRLBM_E_INVALID, "field %d holds %zu values, got %zu", (int)f, want, count);
   CK(c, hipMemcpyAsync(d, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  // uploaded force fields are the caller's: no forcing_GP state goes with them
+  if (f == RRLBM_FIELD_FORCEX || f == RRLBM_FIELD_FORCEY || f == RRLBM_FIELD_FORCEZ)
+    c->force_fields = rrlbm_ctx::Forcing{};
   return RRLBM_OK;
 }
 
@@ -707,6 +737,7 @@ static int init_common(rrlbm_ctx* c, const InitSpec& s) {
   CK(c, ref_init_equilibrium(c->q, c->ref, c->grid, c->stream, is_bgk(c->p)));
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  c->force_fields = rrlbm_ctx::Forcing{};  // the force fields hold the uniform force again
   c->packed = false;
   return RRLBM_OK;
 }
@@ -808,31 +839,41 @@ int rrlbm_graph_capture(rrlbm_ctx* c, int nsteps) {
     return fail(c, RRLBM_E_STATE, "the fused state's last collision took forcing_GP: run a plain fused step first");
   if (nsteps < 1 || nsteps > 100000) return fail(c, RRLBM_E_INVALID, "nsteps must be in [1, 100000]");
   CK(c, hipStreamSynchronize(c->stream));
-  if (c->exec) (void)hipGraphExecDestroy(c->exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->exec = nullptr;
-  c->graph = nullptr;
+  drop_graph(c);
   // the launches record the buffer swaps and the dither counter on the host;
-  // nothing runs during capture, so restore them afterwards
+  // nothing runs during capture, so restore them afterwards.  FP16 with
+  // stochastic rounding: the captured steps take their step number as the
+  // device counter d_step plus their place in the sequence, and the graph
+  // ends by advancing d_step, so every replay rounds with new dither.
   void* const a0 = c->pop_a;
   void* const b0 = c->pop_b;
   void* const ua0 = c->u_a;
   void* const ub0 = c->u_b;
   const uint32_t sr0 = c->sr_step;
+  const bool dev_step = f16_dither(c->p);
+  if (dev_step) {
+    c->capture_dev_step = true;
+    c->sr_step = 0;
+  }
   CK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = fused_steps(c, nsteps);
+  hipError_t e = hipSuccess;
+  if (!rc && dev_step) e = launch_counter_add(c->d_step, (uint32_t)nsteps, c->stream);
   hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (e == hipSuccess) e = e2;
   c->pop_a = a0; c->pop_b = b0; c->u_a = ua0; c->u_b = ub0; c->sr_step = sr0;
+  c->capture_dev_step = false;
   if (rc || e != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     if (rc) return rc;
-    return fail(c, RRLBM_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return fail(c, RRLBM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
   }
   c->graph = g;
   CK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
   c->graph_steps = nsteps;
   c->graph_pop_a = a0;
+  c->graph_dev_step = dev_step;
   return RRLBM_OK;
 }
 
@@ -840,6 +881,14 @@ int rrlbm_graph_launch(rrlbm_ctx* c, int times) {
   NEED(c);
   if (!c->exec) return fail(c, RRLBM_E_STATE, "no captured graph");
   if (times < 0) return fail(c, RRLBM_E_INVALID, "times must be >= 0");
+  if (!c->packed) return fail(c, RRLBM_E_STATE, "no fused state (rrlbm_fused_pack must run before graph replays)");
+  // the captured steps complete a step collided under the uniform force
+  if (c->force_cur.gp)
+    return fail(c, RRLBM_E_STATE, "the fused state's last collision took forcing_GP: run a plain fused step first");
+  if (times > 0 && c->graph_dev_step) {
+    hipError_t e = launch_counter_set(c->d_step, c->sr_step, c->stream);
+    if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "step counter launch failed: %s", hipGetErrorString(e));
+  }
   for (int t = 0; t < times; ++t) {
     if (c->pop_a != c->graph_pop_a)
       return fail(c, RRLBM_E_STATE, "the state sits in the other buffer than at capture (odd step count); capture again");
@@ -849,6 +898,7 @@ int rrlbm_graph_launch(rrlbm_ctx* c, int times) {
       if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
     }
     c->sr_step += (uint32_t)c->graph_steps;
+    c->force_cur = rrlbm_ctx::Forcing{};
   }
   return RRLBM_OK;
 }
@@ -938,6 +988,7 @@ int rrlbm_read_restart(rrlbm_ctx* c, const char* dir, int istep) {
   // initial.cu:262: the force fields are re-imposed after a restart
   CK(c, ref_forcing(c->ref, c->grid, c->p.force[0], c->p.force[1], c->p.force[2], c->stream));
   CK(c, hipStreamSynchronize(c->stream));
+  c->force_fields = rrlbm_ctx::Forcing{};
   c->packed = false;
   return RRLBM_OK;
 }
@@ -971,6 +1022,7 @@ std::string tune_key(const rrlbm_params& p) {
 
 int realloc_pops(rrlbm_ctx* c, size_t bytes) {
   CK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
   dfree(c->pop_a);
   dfree(c->pop_b);
   c->pop_a = c->pop_b = nullptr;
@@ -994,6 +1046,7 @@ int apply_layout(rrlbm_ctx* c, int group_rows, int pad) {
     int rc = realloc_pops(c, l.bytes);
     if (rc) return rc;
   }
+  if (memcmp(&l, &c->lay, sizeof l) != 0) drop_graph(c);  // captured launches carry the old layout
   c->lay = l;
   c->pop_bytes = l.bytes;
   c->p.pop_group_rows = p.pop_group_rows;
@@ -1160,11 +1213,13 @@ struct rrlbm_batch {
   int wg_per_cu = 0;            // 0: one workgroup per tile; k: persistent grid of k per CU
   int cus = 256;
   uint32_t parity = 0;  // 1: every instance's current state sits in its table dst buffer
-  uint32_t sr_step = 0;  // eager batched / sequential steps taken (FP16 rounding dither)
+  uint32_t sr_step = 0;  // batched / sequential steps taken (FP16 rounding dither)
+  uint32_t* d_step = nullptr;  // device step counter of captured FP16 steps
   bool ready = false;   // instances initialised and packed, table uploaded
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
+  bool graph_dev_step = false;  // the captured steps read d_step (advanced by every replay)
   uint32_t graph_parity = 0;
   int64_t cells = 0;
   std::string err;
@@ -1228,7 +1283,7 @@ uint32_t batch_grid(const rrlbm_batch* b) {
 
 int batch_launch(rrlbm_batch* b) {
   hipError_t e = launch_batched(b->storage, b->d_table, b->d_work, b->total_blocks, batch_grid(b), b->parity,
-                                b->sr_step, b->stream);
+                                b->sr_step, nullptr, b->stream);
   if (e != hipSuccess) return bfail(b, RRLBM_E_HIP, "batched launch failed: %s", hipGetErrorString(e));
   ++b->sr_step;
   batch_swap_all(b);
@@ -1253,11 +1308,13 @@ int batch_sequential(rrlbm_batch* b, int nsteps) {
 
 int batch_graph_launch(rrlbm_batch* b, int times) {
   if (!b->exec) return bfail(b, RRLBM_E_STATE, "no captured graph");
+  if (times > 0 && b->graph_dev_step) BCK(b, launch_counter_set(b->d_step, b->sr_step, b->stream));
   for (int t = 0; t < times; ++t) {
     if (b->parity != b->graph_parity)
       return bfail(b, RRLBM_E_STATE, "the graph was captured at the other buffer parity; capture again");
     BCK(b, hipGraphLaunch(b->exec, b->stream));
     for (int s = 0; s < b->graph_steps; ++s) batch_swap_all(b);
+    b->sr_step += (uint32_t)b->graph_steps;
   }
   return RRLBM_OK;
 }
@@ -1304,6 +1361,8 @@ int rrlbm_batch_create(const rrlbm_instance* inst, int count, int storage, int k
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e == hipSuccess) e = hipMalloc(&b->d_table, count * sizeof(FusedArgs));
   if (e == hipSuccess) e = hipMalloc(&b->d_work, work.size() * sizeof(uint2));
+  if (e == hipSuccess) e = hipMalloc(&b->d_step, 256);
+  if (e == hipSuccess) e = hipMemsetAsync(b->d_step, 0, 256, b->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->d_work, work.data(), work.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -1320,6 +1379,7 @@ int rrlbm_batch_destroy(rrlbm_batch* b) {
   for (rrlbm_ctx* c : b->inst) rrlbm_destroy(c);
   dfree(b->d_table);
   dfree(b->d_work);
+  dfree(b->d_step);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1390,23 +1450,29 @@ int rrlbm_batch_graph_capture(rrlbm_batch* b, int nsteps) {
     b->graph = nullptr;
   }
   const uint32_t start = b->parity;
+  // FP16 (stochastic rounding): step numbers relative to the device counter,
+  // which the graph advances at its end, so that replays take new dither
+  const bool dev_step = b->storage == RRLBM_STORE_F16;
   BCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   uint32_t par = start;
-  for (int s = 0; s < nsteps; ++s) {
-    hipError_t e = launch_batched(b->storage, b->d_table, b->d_work, b->total_blocks, batch_grid(b), par,
-                                  b->sr_step + (uint32_t)s, b->stream);
-    if (e != hipSuccess) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(b->stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      return bfail(b, RRLBM_E_HIP, "launch during capture failed: %s", hipGetErrorString(e));
-    }
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+    e = launch_batched(b->storage, b->d_table, b->d_work, b->total_blocks, batch_grid(b), par,
+                       dev_step ? (uint32_t)s : b->sr_step + (uint32_t)s, dev_step ? b->d_step : nullptr, b->stream);
     par ^= 1u;
+  }
+  if (e == hipSuccess && dev_step) e = launch_counter_add(b->d_step, (uint32_t)nsteps, b->stream);
+  if (e != hipSuccess) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(b, RRLBM_E_HIP, "launch during capture failed: %s", hipGetErrorString(e));
   }
   BCK(b, hipStreamEndCapture(b->stream, &b->graph));
   BCK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   b->graph_steps = nsteps;
   b->graph_parity = start;
+  b->graph_dev_step = dev_step;
   return RRLBM_OK;
 }
 

@@ -29,6 +29,15 @@ Fixtures (plain text, data only):
       the x profile of u_y at each step (all digits) and, in the header, the
       largest deviation of any node from its profile and the largest |u_x|,
       |u_z|, so a test can check the whole field.
+  cavity_128x128x2_re1000_it5000_lines.txt
+      the reference's lid-driven cavity solver (Cavity/*.f90, D3Q27 RR,
+      128x128x2, walls on the x and y faces, lid x = LX moving in +y at 0.1,
+      Re 1000) compiled from source and run by oracle/build_oracle.py to its
+      first field dump (step 5000, Fortran unformatted rho, ux, uy, uz): rho,
+      ux, uy along the lines x = 0, 1, 63, 64, 126, 127 and y = 0, 1, 63, 64,
+      126, 127 of the z = 0 layer (all digits; 0-based indices, x fastest in
+      the reference's arrays), with the largest |uz| and the largest
+      difference between the two z layers in the header.
 """
 import os
 import sys
@@ -133,6 +142,29 @@ def laminar_profiles(dst, nx=60, ny=20, nz=2):
     np.savetxt(dst, np.column_stack([np.arange(nx)] + prof), fmt=["%d", "%.17e", "%.17e", "%.17e"], header=hdr)
 
 
+CAVITY_LINES = (0, 1, 63, 64, 126, 127)
+
+
+def cavity_reference_lines(dst, n=128, nz=2, step=5000):
+    import build_oracle
+    dump = build_oracle.build_reference_cavity(run=True)
+    f = {q: read_fortran_record(os.path.join(dump, "%s%d.dat" % (q, step)), n * n * nz).reshape(nz, n, n)
+         for q in ("rho", "ux", "uy", "uz")}
+    layers = max(float(np.abs(f[q][1] - f[q][0]).max()) for q in f)
+    rows = []
+    for axis in (0, 1):  # 0: the line x = k along y; 1: the line y = k along x
+        for k in CAVITY_LINES:
+            for j in range(n):
+                x, y = (k, j) if axis == 0 else (j, k)
+                rows.append((axis, k, j, f["rho"][0, y, x], f["ux"][0, y, x], f["uy"][0, y, x]))
+    hdr = ("reference Cavity solver (Cavity/*.f90 built from source, amdflang -O0 -fdefault-real-8), 128x128x2, "
+           "D3Q27 RR, Re 1000, lid x = LX moving in +y at 0.1, fields after %d steps, layer z = 0\n"
+           "max |uz| = %.3e, max difference between the z layers = %.3e\n"
+           "columns: axis (0: line x = k, running y; 1: line y = k, running x) k j rho ux uy"
+           % (step, float(np.abs(f["uz"]).max()), layers))
+    np.savetxt(dst, np.array(rows), fmt=["%d", "%d", "%d", "%.17e", "%.17e", "%.17e"], header=hdr)
+
+
 def main():
     fortran_midline(REF_TG27, os.path.join(HERE, "tg_d3q27_L100_it20000_fortran_midline.txt"),
                     "from Taylor_Green/RRBMa0.01L100.txt (reference output)")
@@ -146,6 +178,7 @@ def main():
     tg2d_c1_ke(os.path.join(HERE, "tg2d_c1_ke_oracle.txt"))
     laminar_profiles(os.path.join(HERE, "laminar_60x20x2_uy_profiles.txt"))
     cavity_ghia(HERE)
+    cavity_reference_lines(os.path.join(HERE, "cavity_128x128x2_re1000_it5000_lines.txt"))
     re180_profile(os.path.join(HERE, "re180_spectral_uplus.txt"))
 
 

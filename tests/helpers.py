@@ -63,3 +63,26 @@ def fortran_to_cuda_order(f_fortran_aos, L, LZ):
         jf = int(np.where((cF == cC[ic]).all(axis=1))[0][0])
         out[ic] = aos[:, jf]
     return out
+
+
+CAVITY_FIXTURE = "cavity_128x128x2_re1000_it5000_lines.txt"
+# the reference Cavity solver's configuration (Cavity/para.f90:25-28, var_inc.f90:5,20):
+# 128 x 128 x 2, lid speed 0.1 in +y on the x = LX face, Re 1000
+CAVITY_N, CAVITY_LID, CAVITY_RE, CAVITY_STEPS = 128, 0.1, 1000.0, 5000
+
+
+def cavity_tau():
+    return 3.0 * (CAVITY_LID * CAVITY_N / CAVITY_RE) + 0.5
+
+
+def cavity_fixture_errors(rho, ux, uy, golden_dir):
+    """Largest |difference| of rho, ux, uy (reference layout, N = 128*128*2)
+    from the reference Cavity solver's step-5000 fields on the fixture's lines."""
+    import os
+    a = np.loadtxt(os.path.join(golden_dir, CAVITY_FIXTURE))
+    n = CAVITY_N
+    axis, k, j = a[:, 0].astype(int), a[:, 1].astype(int), a[:, 2].astype(int)
+    x = np.where(axis == 0, k, j)
+    y = np.where(axis == 0, j, k)
+    idx = y * n + x  # layer z = 0
+    return tuple(float(np.abs(np.asarray(got)[idx] - a[:, c]).max()) for got, c in ((rho, 3), (ux, 4), (uy, 5)))

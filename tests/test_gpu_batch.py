@@ -22,14 +22,22 @@ def fields_of(b, k, rr):
     return s.download(rr.Field.F), s.macroscopic()
 
 
+# with a 96 x 64 x 40 box the work list has 1077 tiles: more than the 256 and
+# 768 workgroups of persistent grids of 1 and 3 per compute unit on an MI355X
+INSTANCES_BIG = INSTANCES + [(96, 64, 40, 0.56, 0.05)]
+
+
 @pytest.mark.parametrize("storage", [0, 1, 2])
 def test_batched_equals_sequential_and_graph(rr, storage):
     """FP16: the stochastic-rounding dither depends on (cell, step, instance)
-    only, so the three launch forms stay bit-identical."""
+    only, so the launch forms stay bit-identical; the graph of 2 steps is
+    replayed three times before one eager step, so its replays must advance
+    the dither."""
     steps = 7
     out = {}
+    inst = INSTANCES_BIG
     for mode in ("batched", "sequential", "graph", "persistent1", "persistent3", "autotuned"):
-        with rr.Batch(INSTANCES, storage=storage, keep_reference=True) as b:
+        with rr.Batch(inst, storage=storage, keep_reference=True) as b:
             if mode.startswith("persistent"):
                 b.set_grid(int(mode[-1]))   # 1 or 3 workgroups per CU walking the work list
             if mode == "autotuned":
@@ -41,10 +49,11 @@ def test_batched_equals_sequential_and_graph(rr, storage):
             elif mode == "sequential":
                 b.step_sequential(steps)
             else:
-                b.graph_capture(steps)
-                b.graph_launch(1)
-            out[mode] = [fields_of(b, k, rr) for k in range(len(INSTANCES))]
-    for k in range(len(INSTANCES)):
+                b.graph_capture(2)
+                b.graph_launch(3)
+                b.step(1)
+            out[mode] = [fields_of(b, k, rr) for k in range(len(inst))]
+    for k in range(len(inst)):
         for mode in ("sequential", "graph", "persistent1", "persistent3", "autotuned"):
             assert np.array_equal(out["batched"][k][0], out[mode][k][0]), (k, mode)
 

@@ -44,6 +44,38 @@ def test_f16_stochastic_rounding_keeps_the_laminar_acceleration(rr):
     assert err_rn > 5 * err_sr, (err_rn, err_sr)
 
 
+def bulk_history_graph(rr, storage, blocks=4, replays=50, graph_steps=100):
+    """As bulk_history, stepping by replays of a captured graph of 100 steps
+    (pack, then blocks x replays x 100 steps)."""
+    from rrlbm import configs
+
+    s, meta = configs.channel(storage=storage, lx=32, ly=64, lz=16, les=False, amp_streak=0.0, amp_noise=0.0,
+                              f16_rounding=0)
+    with s:
+        u0 = s.diag_flow()["u_sum"][0] / s.n / meta["u_tau"]
+        s.fused_pack()
+        s.graph_capture(graph_steps)
+        out = []
+        for _ in range(blocks):
+            s.graph_launch(replays)
+            out.append(s.fused_diag()["u_sum"][0] / s.n / meta["u_tau"])
+    return u0, np.array(out)
+
+
+def test_f16_graph_replays_keep_the_laminar_acceleration(rr):
+    """The laminar channel of the test above, 20 000 steps as 200 replays of
+    one captured 100-step graph: FP16 with stochastic rounding stays within
+    the eager run's bound of FP64 (each replay takes new dither from the
+    device step counter; a dither repeated every 100 steps is a correlated
+    rounding error)."""
+    u0, ref = bulk_history_graph(rr, 0)
+    _, gr = bulk_history_graph(rr, 2)
+    gain = ref - u0
+    err = np.abs(gr - ref).max() / gain[-1]
+    print("FP16 graph replays: relative error of the U_b+ gain %.2e" % err)
+    assert err < 5e-3, (err, gr, ref)
+
+
 def test_f16_rounding_is_reproducible(rr):
     """The dither is a fixed function of (cell, population, step): two runs of
     the same steps give bit-identical states; the two rounding modes differ."""

@@ -174,15 +174,22 @@ def test_layout_parameters_are_reported_and_checked(rr):
         assert ei.value.status == 1
 
 
+# 128 x 64 x 40 = 327 680 cells = 1280 blocks of 256 (640 of the two-cell
+# kernel): more blocks than a persistent grid of 1 or 3 workgroups per compute
+# unit has workgroups on the 256 CUs of an MI355X, so those grids stride.
+PERSIST_DIMS = (128, 64, 40)
+
+
 @pytest.mark.parametrize("shape", [dict(persistent_wg_per_cu=1), dict(persistent_wg_per_cu=3, xcd_remap=1),
                                    dict(cells_per_thread=2, persistent_wg_per_cu=1), dict(xcd_remap=1),
                                    dict(xcd_remap=4), dict(xcd_remap=16, persistent_wg_per_cu=1),
                                    dict(cells_per_thread=2, xcd_remap=2)])
 def test_launch_shapes_are_bit_identical(rr, shape):
-    """Persistent grids (workgroups striding over the blocks, diagnostics
-    included), the XCD-remapped block order and two cells per thread compute
-    the same state as one workgroup per block."""
-    g = channel_state(oracle.D3Q27_CUDA, 64, 24, 9, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 0, 0))
+    """Persistent grids (workgroups striding over the blocks; the unpack and
+    diagnostics passes keep one workgroup per block), the XCD-remapped block
+    order and two cells per thread compute the same state as one workgroup
+    per block."""
+    g = channel_state(oracle.D3Q27_CUDA, *PERSIST_DIMS, u_tau=0.01, visc=0.002, seed=9, force=(2e-5, 0, 0))
     tau = 0.5 + 3 * 0.002
     kw = dict(storage=1, tau=tau, force=(2e-5, 0, 0), wall_y=1, wall_rr=1)
     f0, m0, d0 = run_fused(rr, g, 12, **kw)
@@ -194,6 +201,25 @@ def test_launch_shapes_are_bit_identical(rr, shape):
         return
     assert np.array_equal(f0, f1) and np.array_equal(m0[1], m1[1])
     assert d0["kinetic_energy"] == d1["kinetic_energy"] and d0["mass_excess"] == d1["mass_excess"]
+
+
+@pytest.mark.parametrize("les_lagged", [0, 1])
+def test_persistent_grids_cover_every_pass(rr, les_lagged):
+    """With LES in the reference order (a velocity pass, then the step) or
+    lagged, persistent grids of 1 and 4 workgroups per compute unit on a grid
+    of more blocks than that (1280 against 256 and 1024 workgroups) give, bit
+    for bit, the populations, fields and diagnostics of one workgroup per
+    block: every pass covers every block."""
+    g = channel_state(oracle.D3Q27_CUDA, *PERSIST_DIMS, u_tau=0.01, visc=0.002, seed=19, force=(2e-5, 0, 0))
+    kw = dict(storage=1, tau=0.5 + 3 * 0.002, force=(2e-5, 0, 0), wall_y=1, wall_rr=1, use_les=1,
+              les_lagged=les_lagged)
+    f0, m0, d0 = run_fused(rr, g, 6, persistent_wg_per_cu=0, **kw)
+    for k in (1, 4):
+        f1, m1, d1 = run_fused(rr, g, 6, persistent_wg_per_cu=k, **kw)
+        assert np.array_equal(f0, f1), k
+        for a, b in zip(m0, m1):
+            assert np.array_equal(a, b), k
+        assert d0 == d1, k
 
 
 def test_autotune_picks_a_shape_with_identical_results(rr):
@@ -226,14 +252,16 @@ def test_autotune_picks_a_shape_with_identical_results(rr):
         assert t2["candidates"] == 0 and t2["pop_group_rows"] == t["pop_group_rows"]
 
 
+@pytest.mark.parametrize("storage", [1, 2])
 @pytest.mark.parametrize("les", [0, 1])
-def test_graph_replays_match_eager_steps(rr, les):
+def test_graph_replays_match_eager_steps(rr, les, storage):
     """Fused steps captured into a HIP graph: three replays of 4 steps, with
     eager work and synchronisation between them, give the state of 12 eager
-    steps bit for bit (FP32, with and without the lagged LES velocity
-    buffers); an odd capture length refuses its second replay."""
+    steps bit for bit (FP32, and FP16 with stochastic rounding, whose replays
+    take new dither from the device step counter; with and without the lagged
+    LES velocity buffers); an odd capture length refuses its second replay."""
     g = channel_state(oracle.D3Q27_CUDA, 64, 12, 8, u_tau=0.01, visc=0.002, seed=21, force=(2e-5, 0, 0))
-    kw = dict(storage=1, tau=0.5 + 3 * 0.002, force=(2e-5, 0, 0), wall_y=1, wall_rr=1, use_les=les)
+    kw = dict(storage=storage, tau=0.5 + 3 * 0.002, force=(2e-5, 0, 0), wall_y=1, wall_rr=1, use_les=les)
     f_eager, m_eager, d_eager = run_fused(rr, g, 13, **kw)
     with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, **kw) as s:
         upload_state(s, g)
@@ -251,6 +279,24 @@ def test_graph_replays_match_eager_steps(rr, les):
         with pytest.raises(rr.RRLBMError) as ei:
             s.graph_launch(1)
         assert ei.value.status == 3
+        if storage == 1 and not les:
+            # a replay completes a step collided under the uniform force: refused
+            # after forcing_GP steps until a plain step has run
+            s.graph_capture(2)
+            s.fused_step_gp(0, 0.3)
+            s.fused_step_gp(1, 0.3)
+            with pytest.raises(rr.RRLBMError) as ei:
+                s.graph_launch(1)
+            assert ei.value.status == 3
+            s.fused_step(2)
+            s.graph_launch(1)
+            # a new population layout (autotuning) drops the captured graph
+            s.init_rest()
+            s.autotune(trial_steps=1)
+            s.fused_pack()
+            with pytest.raises(rr.RRLBMError) as ei:
+                s.graph_launch(1)
+            assert ei.value.status == 3
 
 
 @pytest.mark.parametrize("force", [(2e-5, 1e-6, 0), (2e-5, 0, 0)])
