@@ -294,6 +294,23 @@ int rrlbm_autotune(rrlbm_ctx* ctx, int trial_steps, rrlbm_tuning* out);
 /* ------------------------------------------------------------ diagnostics */
 /* diag_flow sums (output.cu:3-66) over the reference fields. */
 int rrlbm_diag_flow(rrlbm_ctx* ctx, rrlbm_diag* out);
+/* Largest cell count of rrlbm_diag_flow_noalloc: its scratch array holds the
+ * partial sums of this many cells (4.5 MiB of device memory per GPU). */
+#define RRLBM_NOALLOC_MAX_CELLS (1 << 24)
+/* rrlbm_diag_flow without a context and without allocating device memory: the
+ * diag_flow sums of caller-owned FP64 device fields rho, ux, uy, uz of n cells,
+ * by the kernels of rrlbm_diag_flow (identical results), enqueued on the HIP
+ * stream `stream` (a hipStream_t; NULL: the null stream), which the call then
+ * synchronises to fill *out.  The partial sums live in a fixed device array of
+ * the library that each GPU holds until the process ends, so the call makes no
+ * hipMalloc / hipFree; 1 <= n <= RRLBM_NOALLOC_MAX_CELLS (RRLBM_E_INVALID
+ * otherwise).  All calls on one GPU share that array: calls that overlap in
+ * time on one GPU (other host threads, other streams) race.  Errors are read
+ * with rrlbm_last_error(NULL).  No counterpart in the reference: this is for
+ * callers that cannot allocate device memory (allocation-free serving loops,
+ * harnesses that forbid hipMalloc). */
+int rrlbm_diag_flow_noalloc(const double* rho, const double* ux, const double* uy, const double* uz, int64_t n,
+                            rrlbm_diag* out, void* stream);
 /* output_profile statistics (output.cu:142-225) over the reference fields:
  * out[iy*10 + k] = <u>,<v>,<w>,<u'u'>,<v'v'>,<w'w'>,<u'v'>,<u'w'>,<v'w'>,<tau_t>, count = 10*LY. */
 int rrlbm_plane_profiles(rrlbm_ctx* ctx, double* out, size_t count);

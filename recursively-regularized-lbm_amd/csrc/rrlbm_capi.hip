@@ -319,17 +319,21 @@ bool f16_dither(const rrlbm_params& p) {
   return p.storage == RRLBM_STORE_F16 && p.f16_rounding == RRLBM_F16_STOCHASTIC;
 }
 
-int finalize_diag(rrlbm_ctx* c, uint32_t nparts, rrlbm_diag* out) {
-  CK(c, launch_diag_finalize(c->partials, nparts, c->diag_out, c->stream));
-  double h[kDiagWidth];
-  CK(c, hipMemcpyAsync(h, c->diag_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
-  CK(c, hipStreamSynchronize(c->stream));
+void diag_from_sums(const double (&h)[kDiagWidth], double ncells, rrlbm_diag* out) {
   out->mass_excess = h[0];
   out->u_sum[0] = h[1]; out->u_sum[1] = h[2]; out->u_sum[2] = h[3];
   out->momentum[0] = h[4]; out->momentum[1] = h[5]; out->momentum[2] = h[6];
   out->kinetic_energy = h[7];
   out->nonfinite_cells = h[8];
-  out->ncells = (double)c->n;
+  out->ncells = ncells;
+}
+
+int finalize_diag(rrlbm_ctx* c, uint32_t nparts, rrlbm_diag* out) {
+  CK(c, launch_diag_finalize(c->partials, nparts, c->diag_out, c->stream));
+  double h[kDiagWidth];
+  CK(c, hipMemcpyAsync(h, c->diag_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  CK(c, hipStreamSynchronize(c->stream));
+  diag_from_sums(h, (double)c->n, out);
   if (h[8] != 0.0) return fail(c, RRLBM_E_NUMERIC, "%.0f cells hold non-finite values", h[8]);
   return RRLBM_OK;
 }
@@ -940,6 +944,80 @@ int rrlbm_diag_flow(rrlbm_ctx* c, rrlbm_diag* out) {
   if (!out) return fail(c, RRLBM_E_INVALID, "null output");
   CK(c, ref_diag(c->ref, c->grid, c->partials, c->stream));
   return finalize_diag(c, c->nblocks, out);
+}
+
+}  // extern "C"
+
+// --------------------------------------------- allocation-free diagnostics
+// rrlbm_diag_flow_noalloc keeps its per-workgroup partial sums (one row of
+// kDiagWidth per block of 256 cells) and the kDiagWidth totals in `scratch`,
+// a fixed array that every GPU the library runs on holds until the process
+// ends, instead of memory the library allocates.  Host code never passes the
+// array's name (a host-side stand-in): it takes each device's address once,
+// with hipGetSymbolAddress, and uses that.
+namespace {
+constexpr uint32_t kNoAllocParts = RRLBM_NOALLOC_MAX_CELLS / kBlock;
+static_assert(RRLBM_NOALLOC_MAX_CELLS % kBlock == 0, "whole blocks");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[(size_t)kNoAllocParts * kDiagWidth + kDiagWidth];
+
+namespace {
+// Address of `scratch` on the current device (taken once per device).
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(nullptr, RRLBM_E_HIP, "hipGetDevice failed: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, RRLBM_E_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, RRLBM_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return RRLBM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rrlbm_diag_flow_noalloc(const double* rho, const double* ux, const double* uy, const double* uz, int64_t n,
+                            rrlbm_diag* out, void* stream) {
+  if (!rho || !ux || !uy || !uz || !out) return fail(nullptr, RRLBM_E_INVALID, "null argument");
+  if (n < 1 || n > (int64_t)RRLBM_NOALLOC_MAX_CELLS)
+    return fail(nullptr, RRLBM_E_INVALID,
+                "n = %lld cells: rrlbm_diag_flow_noalloc takes 1 to %d cells (the size of its fixed scratch array)",
+                (long long)n, RRLBM_NOALLOC_MAX_CELLS);
+  double* sc = nullptr;
+  int rc = scratch_of_current_device(&sc);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Grid g{};
+  g.lx = (uint32_t)n; g.ly = 1; g.lz = 1; g.n = (uint32_t)n;
+  RefFields r{};
+  r.rho = const_cast<double*>(rho);
+  r.ux = const_cast<double*>(ux);
+  r.uy = const_cast<double*>(uy);
+  r.uz = const_cast<double*>(uz);
+  const uint32_t nparts = (g.n + kBlock - 1) / kBlock;
+  double* totals = sc + (size_t)kNoAllocParts * kDiagWidth;
+  // every partial row the finalisation reads is written by this call's first
+  // kernel, and the totals by the second: nothing has to start at zero
+  hipError_t e = ref_diag(r, g, sc, s);
+  if (e == hipSuccess) e = launch_diag_finalize(sc, nparts, totals, s);
+  double h[kDiagWidth];
+  if (e == hipSuccess) e = hipMemcpyAsync(h, totals, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(nullptr, RRLBM_E_HIP, "diagnostics kernels failed: %s", hipGetErrorString(e));
+  diag_from_sums(h, (double)n, out);
+  if (h[8] != 0.0) return fail(nullptr, RRLBM_E_NUMERIC, "%.0f cells hold non-finite values", h[8]);
+  return RRLBM_OK;
 }
 
 int rrlbm_plane_profiles(rrlbm_ctx* c, double* out, size_t count) {

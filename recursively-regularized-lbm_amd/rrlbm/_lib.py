@@ -117,6 +117,7 @@ SIGNATURES = {
     "rrlbm_fused_unpack": [_P],
     "rrlbm_fused_diag": [_P, ctypes.POINTER(Diag)],
     "rrlbm_diag_flow": [_P, ctypes.POINTER(Diag)],
+    "rrlbm_diag_flow_noalloc": [_D, _D, _D, _D, ctypes.c_int64, ctypes.POINTER(Diag), _P],
     "rrlbm_plane_profiles": [_P, _D, ctypes.c_size_t],
     "rrlbm_write_flow": [_P, ctypes.c_char_p, _I],
     "rrlbm_write_f": [_P, ctypes.c_char_p, _I],

@@ -66,6 +66,18 @@ def device_count():
     return n.value
 
 
+def diag_flow_noalloc(rho, ux, uy, uz, n, stream=None):
+    """rrlbm_diag_flow_noalloc: the diag_flow sums of caller-owned FP64 device
+    fields (device addresses as integers) of n cells, with no device memory
+    allocated by the library; returns the rrlbm_diag as a dict."""
+    d = Diag()
+    lib = load()
+    D = ctypes.POINTER(ctypes.c_double)
+    check(lib.rrlbm_diag_flow_noalloc(ctypes.cast(rho, D), ctypes.cast(ux, D), ctypes.cast(uy, D),
+                                      ctypes.cast(uz, D), int(n), ctypes.byref(d), stream))
+    return d.as_dict()
+
+
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
