@@ -415,13 +415,31 @@ def main_validate(args):
     print(json.dumps({"workload": "validate", "results": res}))
 
 
+def glibc_rand():
+    """(srand, draw) of the C library's generator: the reference draws its
+    forcing_GP phases as rand() / RAND_MAX after srand(12345) (main.cu:3-5,30)."""
+    import ctypes
+
+    libc = ctypes.CDLL(None)
+    libc.srand.argtypes = [ctypes.c_uint]
+    libc.rand.restype = ctypes.c_int
+    rand_max = 2147483647  # glibc RAND_MAX
+    return libc.srand, (lambda: libc.rand() / rand_max)
+
+
 def main_turbulence(args):
-    """Long run of the channel DNS (fused path, 384 x 128 x 192) with the
-    reference's protocol (main.cu:29-84: init_arrays, forcing_GP with a new
-    random phase every 2000 steps below step 50 000, then the uniform force)
-    to a statistically steady state; the mean U+ profile over the last window
-    against the spectral Re_tau = 180 profile the reference compares with
-    (Re180.prof), and the bulk velocity against its U_b+ = 15.709."""
+    """Long run of a turbulent channel (fused path) with the reference's
+    protocol (main.cu:29-84): init_arrays (initial.cu:268-325), forcing_GP with
+    a new phase rand()/RAND_MAX every 2000 steps below step 50 000 (srand of
+    --turb-phase-seed, 12345 in the reference), then the uniform force.  Every
+    1000 steps (NDIAG, parameters.h:17) the bulk velocity U_b+ of the diag_flow
+    sums; U_b+ as the reference's notebook averages it (the last 3000 diag
+    samples, turbulent.ipynb:78-106) and over the final window; the mean U+ and
+    the rms profiles (u', v', w', -u'v' in wall units, the two halves folded)
+    over the window, against the spectral Re_tau = 180 profile (Re180.prof).
+    The grid defaults to 384 x 128 x 192 at u_tau = 0.0045; --turb-visc sets
+    the viscosity and u_tau = 2 Re_tau visc / LY as globals.cu:57-63 does (the
+    reference's 64^3 run: --turb-grid 64x64x64 --turb-visc 0.000622)."""
     import numpy as np
 
     import rrlbm
@@ -429,57 +447,90 @@ def main_turbulence(args):
 
     rrlbm.load()
     spec = np.loadtxt(os.path.join(REPO, "tests", "golden", "re180_spectral_uplus.txt"))
-    s, meta = configs.channel(storage=args.turb_storage)
-    u_tau, nu, ly = meta["u_tau"], meta["nu"], s.ly
+    lx, ly, lz = (int(v) for v in args.turb_grid.split("x"))
+    re_tau = 180.0
+    u_tau = 2.0 * re_tau * args.turb_visc / ly if args.turb_visc > 0 else U_TAU
+    extra = {}
+    if args.turb_les:
+        extra["les_lagged"] = 0 if args.turb_les_exact else 1
+    s, meta = configs.channel(storage=args.turb_storage, lx=lx, ly=ly, lz=lz, u_tau=u_tau, les=bool(args.turb_les),
+                              **extra)
+    nu = meta["nu"]
     s.init_arrays(u_tau, nu / u_tau)
-    rng = np.random.default_rng(12345)
+    srand, draw = glibc_rand()
+    srand(args.turb_phase_seed)
     gp_end, phase = args.turb_gp_steps, 0.0
+    ndiag = 1000
     t0 = time.perf_counter()
+    ub = []  # (step, U_b+) every ndiag steps
+
+    def diag(step):
+        d = s.fused_diag()
+        ub.append((step, d["u_sum"][0] / d["ncells"] / u_tau))
+
+    # step 1: forcing_GP(1) then its collision (pack); steps 2 .. gp_end-1 in
+    # the kernel's forcing_GP mode; then the uniform force
     s.forcing_GP(1, phase)
-    s.fused_pack()                       # step 1's collision
-    for istep in range(2, gp_end):
+    s.fused_pack()
+    istep = 1
+    while istep + 1 < min(gp_end, args.turb_steps + 1):
+        istep += 1
         if istep % 2000 == 0:
-            phase = float(rng.uniform())
+            phase = draw()
         s.fused_step_gp(istep, phase)
+        if istep % ndiag == 0:
+            diag(istep)
         if istep % 10000 == 0:
-            s.synchronize()
             log("turbulence: forcing_GP step %d, %.0f s" % (istep, time.perf_counter() - t0))
-    done, acc, cnt, every, chunks = max(gp_end - 1, 1), None, 0, 1000, 0
-    while done < args.turb_steps:
-        n = min(every, args.turb_steps - done)
+    acc, cnt = None, 0
+    while istep < args.turb_steps:
+        n = min(ndiag - istep % ndiag, args.turb_steps - istep)
         s.fused_step(n)
-        done += n
-        chunks += 1
-        if chunks % 50 == 0:
-            s.synchronize()
-            log("turbulence: %d steps, %.0f s" % (done, time.perf_counter() - t0))
-        if done > args.turb_steps - args.turb_window:
+        istep += n
+        if istep % ndiag == 0:
+            diag(istep)
+        if istep % 50000 == 0:
+            log("turbulence: %d steps, %.0f s" % (istep, time.perf_counter() - t0))
+        if istep > args.turb_steps - args.turb_window and istep % ndiag == 0:
             s.fused_unpack()
             p = s.output_profile()
             acc = p.copy() if acc is None else acc + p
             cnt += 1
-    d = s.fused_diag()
     s.close()
+    ubv = np.array([u for _, u in ub])
     prof = acc / cnt
-    # fold the two halves of the channel; cell centres at y + 1/2 from the wall
-    u = 0.5 * (prof[:, 0] + prof[::-1, 0])[: ly // 2]
+
+    def fold(col, sign=1.0):
+        return 0.5 * (prof[:, col] + sign * prof[::-1, col])[: ly // 2]
+
+    u = fold(0)
     yw = np.arange(ly // 2) + 0.5
     yplus = yw * u_tau / nu
     uplus = u / u_tau
     ref = np.interp(yplus, spec[:, 1], spec[:, 2])
     band = (yplus > 5) & (yplus < 150)
-    # friction velocity from the wall gradient of the mean profile (first two cells)
     dudy = (9 * u[0] - u[1]) / 3.0      # quadratic through u = 0 at the half-way wall
-    res = {"workload": "turbulence", "storage": STORAGE_NAME[args.turb_storage], "steps": args.turb_steps,
-           "gp_steps": gp_end, "window": args.turb_window, "samples": cnt,
-           "seconds": round(time.perf_counter() - t0, 1),
-           "ubulk_plus_window": float(prof[:, 0].mean() / u_tau), "ubulk_plus_end": d["u_sum"][0] / d["ncells"] / u_tau,
-           "ubulk_plus_spectral": 15.709, "u_tau_from_wall_gradient": float(np.sqrt(nu * dudy)), "u_tau_imposed": u_tau,
-           "max_abs_dUplus_5_150": float(np.abs(uplus - ref)[band].max()),
-           "max_rel_dUplus_5_150": float((np.abs(uplus - ref) / ref)[band].max()),
+    rnd = lambda a: [round(float(v), 5) for v in a]
+    res = {"workload": "turbulence", "grid": [lx, ly, lz], "storage": STORAGE_NAME[args.turb_storage],
+           "les": ("exact" if args.turb_les_exact else "lagged") if args.turb_les else None,
+           "u_tau": u_tau, "nu": nu, "steps": args.turb_steps, "gp_steps": gp_end, "phase_seed": args.turb_phase_seed,
+           "window": args.turb_window, "samples": cnt, "seconds": round(time.perf_counter() - t0, 1),
+           "ubulk_plus_last3000_diag": float(ubv[-3000:].mean()), "ubulk_plus_window": float(prof[:, 0].mean() / u_tau),
+           "ubulk_plus_spectral": 15.709, "u_tau_from_wall_gradient": float(np.sqrt(nu * dudy)),
+           "max_rel_dUplus_5_150_vs_spectral": float((np.abs(uplus - ref) / ref)[band].max()),
            "centre_uplus": float(uplus[-1]), "centre_uplus_spectral": float(spec[-1, 2]),
-           "profile": {"yplus": [round(float(v), 3) for v in yplus], "uplus": [round(float(v), 4) for v in uplus]}}
-    print(json.dumps(res))
+           "profile": {"yplus": rnd(yplus), "uplus": rnd(uplus),
+                       "urms": rnd(np.sqrt(np.maximum(fold(3), 0)) / u_tau),
+                       "vrms": rnd(np.sqrt(np.maximum(fold(4), 0)) / u_tau),
+                       "wrms": rnd(np.sqrt(np.maximum(fold(5), 0)) / u_tau),
+                       "uv": rnd(-fold(6, -1.0) / u_tau ** 2)},
+           "ubulk_plus_series": [[int(k), round(float(v), 5)] for k, v in ub]}
+    line = json.dumps(res)
+    if args.turb_out:
+        with open(args.turb_out, "w") as fh:
+            fh.write(line + "\n")
+    summary = {k: v for k, v in res.items() if k not in ("profile", "ubulk_plus_series")}
+    print(json.dumps(summary))
 
 
 def main_copy():
@@ -511,6 +562,13 @@ def main():
     ap.add_argument("--turb-window", type=int, default=300000)
     ap.add_argument("--turb-gp-steps", type=int, default=50000)
     ap.add_argument("--turb-storage", type=int, default=1)
+    ap.add_argument("--turb-grid", default="%dx%dx%d" % (LX, LY, LZ))
+    ap.add_argument("--turb-visc", type=float, default=0.0,
+                    help="viscosity; u_tau = 2 Re_tau visc / LY (globals.cu:57-63); 0: u_tau = 0.0045")
+    ap.add_argument("--turb-les", type=int, default=0)
+    ap.add_argument("--turb-les-exact", action="store_true", help="LES in the reference order (velocity pass first)")
+    ap.add_argument("--turb-phase-seed", type=int, default=12345)
+    ap.add_argument("--turb-out", default="", help="write the full result (profiles, U_b+ series) to this file")
     ap.add_argument("--cavity-steps", type=int, default=20000)
     ap.add_argument("--les-steps", type=int, default=50000)
     ap.add_argument("--dns-steps", type=int, default=30000)

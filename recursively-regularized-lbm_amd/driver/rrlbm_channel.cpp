@@ -26,6 +26,9 @@
 //   --rr-order 3|6     RR expansion order (6: the fourth- to sixth-order terms the
 //                      reference keeps commented out in collision.cu / BC.cu)
 //   --bgk              the BGK operator of BGK/collision.cu (plain bounce-back walls)
+//   --phase-seed S     seed of the forcing_GP phases (12345, main.cu:30)
+//   --les-exact        LES in the reference order (a velocity pass, then the step)
+//                      instead of the one-launch lagged-gradient step
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -55,12 +58,9 @@ static double now() {
   return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 
-// Park-Miller minimal standard generator for the forcing_GP phase (the
-// reference draws it from rand() every 2000 steps, main.cu:36-41).
-static double next_phase(unsigned long long* s) {
-  *s = (*s * 48271ull) % 2147483647ull;
-  return (double)*s / 2147483647.0;
-}
+// The forcing_GP phase: the C library's rand() scaled to [0, 1], seeded once
+// with srand (main.cu:3-5,30), drawn every 2000 steps (main.cu:38-41).
+static double random_phase() { return (double)rand() / RAND_MAX; }
 
 int main(int argc, char** argv) {
   rrlbm_params p;
@@ -69,6 +69,7 @@ int main(int argc, char** argv) {
   double re_tau = 180.0, visc = 0.000622;
   bool reference = false, init_channel = false;
   unsigned long long seed = 2024;
+  unsigned phase_seed = 12345;
   std::string dir = "data";
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -102,6 +103,8 @@ int main(int argc, char** argv) {
       const std::string w = next();
       p.wall_rr = w == "fdbc" ? 2 : (w == "bb" ? 0 : 1);
     } else if (a == "--rr-order") p.rr_order = atoi(next());
+    else if (a == "--phase-seed") phase_seed = (unsigned)strtoul(next(), nullptr, 10);
+    else if (a == "--les-exact") p.les_lagged = 0;
     else if (a == "--bgk") {
       p.collision_model = RRLBM_COLLISION_BGK;
       p.wall_rr = 0;
@@ -134,8 +137,8 @@ int main(int argc, char** argv) {
   }
   const long n = (long)p.lx * p.ly * p.lz;
   std::vector<double> prof((size_t)p.ly * 10);
-  unsigned long long rng = 12345;
-  double phase = 0.0;
+  srand(phase_seed);
+  double phase = 0.0;  // d_rand_phase before the first draw (globals.cu:16)
   const double t0 = now();
   double t_out = 0.0;
   bool packed = false;
@@ -143,7 +146,7 @@ int main(int argc, char** argv) {
     // main.cu:33-46: forcing_GP (new phase every 2000 steps) below gp_steps,
     // the uniform force after
     const bool gp = istep < gp_steps;
-    if (gp && istep % 2000 == 0) phase = next_phase(&rng);
+    if (gp && istep % 2000 == 0) phase = random_phase();
     if (reference || !packed) {
       if (gp) CALL(ctx, rrlbm_forcing_gp(ctx, istep, phase));
       else CALL(ctx, rrlbm_forcing(ctx));

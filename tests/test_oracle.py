@@ -293,6 +293,229 @@ def test_init_arrays_properties():
     assert np.allclose(UX.mean(axis=(0, 2)), mean, rtol=1e-12)
 
 
+def _mode_fit(field, basis):
+    """Least-squares amplitude of `basis` in `field` over the x-z plane and the
+    largest residual |field - a basis|."""
+    a = float((field * basis).sum() / (basis * basis).sum())
+    return a, float(np.abs(field - a * basis).max())
+
+
+def test_forcing_gp_bands_are_single_divergence_free_modes():
+    """forcing_GP (macrovar.cu:87-112), lower band (h0 < iy < LYq + h0): f_x - F,
+    f_y and f_z are each one Fourier mode in x and z -- sin(k_x x) cos(k_z z),
+    cos(k_x x) cos(k_z z), cos(k_x x) sin(k_z z) with k_x = 2 pi 3/LX,
+    k_z = 2 pi 2/LZ -- whose y-amplitudes are A_x sin(k_y t), A_y (1 - cos(k_y t)),
+    A_z sin(k_y t) in t = iy - h0, k_y = 2 pi/LYq; the field is divergence-free,
+    A_x k_x + A_y k_y + A_z k_z = 0 (-2 pi + pi + pi), and A(istep) follows
+    sin(2 pi istep/2000).  A swapped sin/cos, a wrong wavenumber, a shifted band
+    row or a wrong coefficient in any component fails one of these."""
+    lx, ly, lz = 48, 32, 16
+    h0, lyq = 2, ly // 4
+    kx, kz, ky = 2 * np.pi * 3 / lx, 2 * np.pi * 2 / lz, 2 * np.pi / lyq
+    X = np.arange(lx)[None, :]
+    Z = np.arange(lz)[:, None]
+    fin = 1e-5
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    amps = {}
+    for istep in (250, 500):
+        g.forcing_gp(fin, istep, 0.3)
+        FX, FY, FZ = (a.reshape(lz, ly, lx) for a in (g.fx, g.fy, g.fz))
+        rows = [iy for iy in range(ly) if h0 < iy < lyq + h0]
+        assert rows == list(range(h0 + 1, h0 + lyq))
+        A = []
+        for iy in rows:
+            ax, rx = _mode_fit(FX[:, iy, :] - fin, np.sin(kx * X) * np.cos(kz * Z))
+            ay, ry = _mode_fit(FY[:, iy, :], np.cos(kx * X) * np.cos(kz * Z))
+            az, rz = _mode_fit(FZ[:, iy, :], np.cos(kx * X) * np.sin(kz * Z))
+            assert max(rx, ry, rz) < 1e-12 * fin, (iy, rx, ry, rz)
+            A.append((ax, ay, az))
+        A = np.array(A)
+        t = np.array(rows) - h0
+        Ax, Ay, Az = (_mode_fit(A[:, k], b)[0] for k, b in
+                      ((0, np.sin(ky * t)), (1, 1 - np.cos(ky * t)), (2, np.sin(ky * t))))
+        assert np.allclose(A[:, 0], Ax * np.sin(ky * t), rtol=0, atol=1e-12 * fin)
+        assert np.allclose(A[:, 1], Ay * (1 - np.cos(ky * t)), rtol=0, atol=1e-12 * fin)
+        assert np.allclose(A[:, 2], Az * np.sin(ky * t), rtol=0, atol=1e-12 * fin)
+        assert abs(Ax * kx + Ay * ky + Az * kz) < 1e-12 * abs(Ay * ky)      # divergence-free
+        # amplitude ratios LX/beta : LYq/2 : LZ/(2 gamma) with beta = 3, gamma = 2 (sign of f_x: -)
+        assert np.isclose(-Ax / Ay, (lx / 3.0) / (0.5 * lyq), rtol=1e-13)
+        assert np.isclose(Az / Ay, (0.5 * lz / 2.0) / (0.5 * lyq), rtol=1e-13)
+        amps[istep] = Ay
+        # every other row outside the two bands carries the uniform force
+        upper = [iy for iy in range(ly) if 0.75 * ly - h0 < iy < ly - h0]
+        assert upper == list(range(ly - h0 - lyq + 1, ly - h0))
+        other = [iy for iy in range(ly) if iy not in rows and iy not in upper]
+        assert (FX[:, other, :] == fin).all() and (FY[:, other, :] == 0).all() and (FZ[:, other, :] == 0).all()
+    # A(istep) = (100 beta / LY) sin(2 pi istep / 2000): A(500) / A(250) = 1 / sin(pi/4)
+    assert np.isclose(amps[500] / amps[250], np.sqrt(2.0), rtol=1e-13)
+    assert np.isclose(amps[500], fin * (100 * 3.0 / ly) * 0.5 * lyq, rtol=1e-13)
+
+
+def test_forcing_gp_upper_band_is_x_independent():
+    """The upper band (0.75 LY - h0 < iy < LY - h0) takes its x phase as the
+    integer ix / LX (always 0) plus the random phase (macrovar.cu:106-112):
+    exactly x-independent, the z modes cos(k_z z) / sin(k_z z), the mirrored
+    y-profile of the lower band (t = iy - (LY - h0 - LYq)) with the opposite
+    signs, and the phase entering as sin / cos(2 pi 3 phase).  Without the x
+    term its divergence is -2 pi A sin(k_y t) cos(6 pi phase) cos(k_z z), not 0."""
+    lx, ly, lz = 48, 32, 16
+    h0, lyq = 2, ly // 4
+    kz, ky = 2 * np.pi * 2 / lz, 2 * np.pi / lyq
+    Z = np.arange(lz)[:, None] + 0 * np.arange(lx)[None, :]
+    fin = 1e-5
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    lower = {}
+    g.forcing_gp(fin, 500, 0.0)
+    FX, FY, FZ = (a.reshape(lz, ly, lx) for a in (g.fx, g.fy, g.fz))
+    for iy in range(h0 + 1, h0 + lyq):
+        lower[iy - h0] = FY[0, iy, 0]    # x = 0, z = 0: A_y (1 - cos(k_y t))
+    for phase in (0.0, 0.13, 0.3):
+        g.forcing_gp(fin, 500, phase)
+        FX, FY, FZ = (a.reshape(lz, ly, lx) for a in (g.fx, g.fy, g.fz))
+        for iy in range(ly - h0 - lyq + 1, ly - h0):
+            t = iy - (ly - h0 - lyq)
+            for F in (FX, FY, FZ):
+                assert (F[:, iy, :] == F[:, iy, :1]).all()                   # x-independent
+            ay, ry = _mode_fit(FY[:, iy, :], np.cos(kz * Z))
+            az, rz = _mode_fit(FZ[:, iy, :], np.sin(kz * Z))
+            ax, rx = _mode_fit(FX[:, iy, :] - fin, np.cos(kz * Z))
+            assert max(rx, ry, rz) < 1e-12 * fin
+            c6, s6 = np.cos(6 * np.pi * phase), np.sin(6 * np.pi * phase)
+            assert np.isclose(ay, -lower[t] * c6, rtol=1e-12, atol=1e-12 * fin)
+            # A_z / A_y and A_x / A_y as in the lower band, times the phase factors
+            ratio_z = (0.5 * lz / 2.0) / (0.5 * lyq) * np.sin(ky * t) / (1 - np.cos(ky * t))
+            ratio_x = (lx / 3.0) / (0.5 * lyq) * np.sin(ky * t) / (1 - np.cos(ky * t))
+            assert np.isclose(az, -lower[t] * c6 * ratio_z, rtol=1e-12, atol=1e-12 * fin)
+            assert np.isclose(ax, lower[t] * s6 * ratio_x, rtol=1e-12, atol=1e-12 * fin)
+
+
+def test_init_arrays_perturbation_and_populations():
+    """init_arrays (initial.cu:277-324): u_x minus its x-z plane mean is the
+    single oblique mode sin(x9 + z9) at the cell centres (x9 = 2 pi (ix + 1/2)/LX,
+    no cos part: the half-cell phase), with amplitude ccc1 y+ -- linear in y+
+    from the nearer wall, the lower formula up to iy = LY/2 inclusive, with
+    ccc1 = -LX A9 u*/(2 pi ystar cc^2) -- and the populations are the rest
+    state shifted by the force, w_i (1 - 3 F c_ix / 2), whatever u is.  A y+
+    from the wrong wall, a missing half cell, a z-only or x-only mode, or
+    populations at the equilibrium of u fail one of these."""
+    lx, ly, lz = 16, 12, 8
+    visc = 0.000622
+    u0 = 2.0 * 180.0 * visc / ly
+    ystar = visc / u0
+    fin = 2.0 * u0 * u0 / ly
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    g.init_arrays(u0, ystar, fin)
+    c, w, _ = oracle.lattice(oracle.D3Q27_CUDA)
+    expect_f = w[:, None] * (1.0 - 1.5 * fin * c[:, 0:1].astype(float))
+    assert np.abs(g.f - expect_f).max() < 1e-17
+    UX = g.ux.reshape(lz, ly, lx)
+    x9 = 2 * np.pi * (np.arange(lx) + 0.5) / lx
+    z9 = 2 * np.pi * (np.arange(lz) + 0.5) / lz
+    S = np.sin(x9[None, :] + z9[:, None])
+    C = np.cos(x9[None, :] + z9[:, None])
+    slope = []
+    for iy in range(ly):
+        pert = UX[:, iy, :] - UX[:, iy, :].mean()
+        a, res = _mode_fit(pert, S)
+        b, _ = _mode_fit(pert, C)
+        assert res < 1e-15 and abs(b) < 1e-16, (iy, res, b)
+        yplus = (iy + 0.5) / ystar if iy <= ly // 2 else (ly - iy - 0.5) / ystar
+        slope.append(a / yplus)
+    assert np.allclose(slope, slope[0], rtol=1e-12)
+    ccc1 = -lx * 0.1 * np.sqrt(2.0) * u0 / (2 * np.pi * ystar * 60.0 ** 2)
+    assert np.isclose(slope[0], ccc1, rtol=1e-12)
+    assert np.all(g.uy == 0) and np.all(g.uz == 0) and np.all(g.rho == 1.0)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_diag_and_profiles_match_long_double_sums(seed):
+    """orc_diag (diag_flow's sums, output.cu:24-35, with the momentum and kinetic
+    energy the library also reports) and orc_profiles (output_profile's plane
+    statistics, output.cu:182-215: plane means and the covariances of the
+    fluctuations about them, divided by LX*LZ) against numpy long-double sums
+    over random fields.  A dropped term, a covariance about zero instead of the
+    plane mean, a division by the wrong count or a transposed plane index
+    fails at far above the 1e-13 tolerance."""
+    lx, ly, lz = 13, 9, 7
+    rng = np.random.default_rng(seed)
+    g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+    n = g.n
+    g.rho[:] = 1.0 + 0.01 * rng.standard_normal(n)
+    for a in (g.ux, g.uy, g.uz):
+        a[:] = 0.3 + 0.05 * rng.standard_normal(n)
+    g.taut[:] = rng.uniform(0, 0.1, n)
+    L = np.longdouble
+    r, u, v, w = (a.astype(L) for a in (g.rho, g.ux, g.uy, g.uz))
+    want = [u.sum(), v.sum(), w.sum(), r.sum(), (r - 1).sum(), (r * u).sum(), (r * v).sum(), (r * w).sum(),
+            (L(0.5) * r * (u * u + v * v + w * w)).sum()]
+    got = g.diag()
+    for k in range(9):
+        assert abs(got[k] - float(want[k])) <= 1e-13 * max(1.0, abs(float(want[k]))), k
+    prof = g.profiles(with_taut=True)
+    shape = (lz, ly, lx)
+    U, V, W, T = (a.astype(L).reshape(shape) for a in (g.ux, g.uy, g.uz, g.taut))
+    for iy in range(ly):
+        pu, pv, pw, pt = (A[:, iy, :] for A in (U, V, W, T))
+        du, dv, dw = pu - pu.mean(), pv - pv.mean(), pw - pw.mean()
+        cnt = lx * lz
+        ref = [pu.mean(), pv.mean(), pw.mean(), (du * du).sum() / cnt, (dv * dv).sum() / cnt, (dw * dw).sum() / cnt,
+               (du * dv).sum() / cnt, (du * dw).sum() / cnt, (dv * dw).sum() / cnt, pt.mean()]
+        for k in range(10):
+            assert abs(prof[iy, k] - float(ref[k])) <= 1e-13 * max(1e-3, abs(float(ref[k]))), (iy, k)
+    # a plane-constant field has zero covariances: the fluctuations are about the plane mean
+    g.ux[:] = np.repeat(np.arange(ly, dtype=float), lx)[None, :].repeat(lz, axis=0).ravel()
+    assert np.abs(g.profiles()[:, 3]).max() == 0.0
+
+
+def test_init_channel_fields_properties():
+    """The channel configuration's seeded initial field (the library's
+    rrlbm_init_channel; no reference counterpart): rho = 1; without noise, u_x
+    is the wall law U = u_tau U+(y+) (y+ in the viscous layer, ln(y+)/0.41 + 5
+    above 10.8, the two branches meeting there to 4e-3) plus one streak mode
+    0.1 U sin(2 pi k_z (z + 1/2)/LZ) cos(2 pi k_x (x + 1/2)/LX + phi) with
+    k_x, k_z in {2, 3, 4}, and u_y = u_z = 0; the noise on each component is
+    bounded by 1% of U_b, centred, with the variance of a uniform variable and
+    uncorrelated between components."""
+    lx, ly, lz = 48, 32, 40
+    u_tau, visc = 0.0045, 0.0045 * 16 / 180.0
+    ystar = visc / u_tau
+    assert abs(np.log(10.8) / 0.41 + 5.0 - 10.8) < 4e-3
+    iy = np.arange(ly)
+    d = np.where(iy < ly // 2, iy + 0.5, ly - iy - 0.5) / ystar
+    U = u_tau * np.where(d <= 10.8, d, np.log(d) / 0.41 + 5.0)
+    ub = U.mean()
+    X = (np.arange(lx) + 0.5)[None, :]
+    Z = (np.arange(lz) + 0.5)[:, None]
+    for seed in (3, 8, 21):
+        g = oracle.Fields(oracle.D3Q27_CUDA, lx, ly, lz)
+        g.init_channel_fields(u_tau, visc, seed, amp_streak=0.1, amp_noise=0.0)
+        assert np.all(g.rho == 1.0) and np.all(g.uy == 0) and np.all(g.uz == 0)
+        UX = g.ux.reshape(lz, ly, lx)
+        assert np.allclose(UX.mean(axis=(0, 2)), U, rtol=1e-12, atol=0)
+        found = []
+        for kx in (1, 2, 3, 4, 5):
+            for kz in (1, 2, 3, 4, 5):
+                for bx in (np.cos, np.sin):
+                    b = np.sin(2 * np.pi * kz * Z / lz) * bx(2 * np.pi * kx * X / lx)
+                    a, _ = _mode_fit(UX[:, 5, :] - U[5], b)
+                    if abs(a) > 1e-9 * U[5]:
+                        found.append((kx, kz))
+        kx, kz = found[0]
+        assert set(found) == {(kx, kz)} and kx in (2, 3, 4) and kz in (2, 3, 4), found
+        for j in range(ly):
+            pert = UX[:, j, :] - U[j]
+            ac, _ = _mode_fit(pert, np.sin(2 * np.pi * kz * Z / lz) * np.cos(2 * np.pi * kx * X / lx))
+            as_, _ = _mode_fit(pert, np.sin(2 * np.pi * kz * Z / lz) * np.sin(2 * np.pi * kx * X / lx))
+            assert np.isclose(np.hypot(ac, as_), 0.1 * U[j], rtol=1e-12)
+        g.init_channel_fields(u_tau, visc, seed, amp_streak=0.0, amp_noise=0.01)
+        nx_, ny_, nz_ = g.ux - np.repeat(U, lx)[None, :].repeat(lz, axis=0).ravel(), g.uy, g.uz
+        for e in (nx_, ny_, nz_):
+            assert np.abs(e).max() <= 0.01 * ub
+            assert abs(e.mean()) < 4 * 0.01 * ub / np.sqrt(3 * e.size)
+            assert abs(e.var() / (0.01 * ub) ** 2 - 1 / 3) < 0.02
+        assert abs(np.corrcoef(nx_, ny_)[0, 1]) < 0.02 and abs(np.corrcoef(ny_, nz_)[0, 1]) < 0.02
+
+
 def splitmix64(x):
     m = (1 << 64) - 1
     x = (x + 0x9E3779B97F4A7C15) & m
