@@ -86,23 +86,54 @@ def traffic_from_profiles(storage):
     return t.get(STORAGE_NAME[storage], {}).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(steps=20):
-    """Oracle (C FP64, OpenMP) reference-order steps on the full grid (about
-    10 s of host work at ~17 MLUPS on 16 cores)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_mlups(make, steps, threads=None):
+    """MLUPS of `steps` reference-order oracle steps on the Fields that make()
+    returns, with `threads` OpenMP threads (None: the host's default)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    g = oracle.Fields(oracle.D3Q27_CUDA, LX, LY, LZ)
-    g.init_channel_fields(U_TAU, VISC, SEED, 0.1, 0.01)
-    g.init_equilibrium()
-    g.set_force((FORCE, 0.0, 0.0))
-    t0 = time.perf_counter()
-    g.step(steps, TAU, walls=(0, 1, 0), wall_rr=1)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": LX * LY * LZ * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": "%d reference-order steps (FP64 oracle, C/OpenMP) of the full %dx%dx%d grid, %.1f s"
-                      % (steps, LX, LY, LZ, dt)}
+    oracle.load()
+    default = oracle.get_threads()
+    oracle.set_threads(threads or default)
+    try:
+        g, run = make(oracle)
+        used = oracle.get_threads()
+        t0 = time.perf_counter()
+        run(g, steps)
+        dt = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(default)
+    return g.n * steps / dt / 1e6, dt, used
+
+
+def cpu_baseline(steps=20, steps_1t=2):
+    """Oracle (C FP64, OpenMP) reference-order steps on the full grid: `steps`
+    steps on all host threads (about 10 s at ~17 MLUPS on 16 threads) and
+    `steps_1t` on one thread (about 10 s)."""
+
+    def make(oracle):
+        g = oracle.Fields(oracle.D3Q27_CUDA, LX, LY, LZ)
+        g.init_channel_fields(U_TAU, VISC, SEED, 0.1, 0.01)
+        g.init_equilibrium()
+        g.set_force((FORCE, 0.0, 0.0))
+        return g, lambda g, n: g.step(n, TAU, walls=(0, 1, 0), wall_rr=1)
+
+    v, dt, used = oracle_mlups(make, steps)
+    v1, dt1, _ = oracle_mlups(make, steps_1t, threads=1)
+    return {"value": v, "unit": "MLUPS", "cores": used, "kind": "port",
+            "sample": "%d reference-order steps (FP64 oracle, C/OpenMP) of the full %dx%dx%d grid, %.1f s on %d "
+                      "threads; %d steps on 1 thread, %.1f s" % (steps, LX, LY, LZ, dt, used, steps_1t, dt1),
+            "value_1_thread": v1, "cpu_model": cpu_model(), "host_threads": os.cpu_count()}
 
 
 def run_ensemble(rrlbm, steps, warmup, storage=1):
@@ -229,9 +260,9 @@ def tg_cpu_baseline(steps=2000):
     t0 = time.perf_counter()
     g.step(steps, 3 * nu + 0.5, walls=(0, 0, 0), wall_rr=0)
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(40000 * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": "%d steps of 100x100x4 (FP64 oracle, C/OpenMP), %.1f s" % (steps, dt)}
+    return {"value": round(40000 * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": oracle.get_threads(),
+            "kind": "port", "sample": "%d steps of 100x100x4 (FP64 oracle, C/OpenMP), %.1f s" % (steps, dt),
+            "cpu_model": cpu_model()}
 
 
 def log(msg):
@@ -246,6 +277,32 @@ def main_configs(args):
 
     rrlbm.load()
     out = []
+
+    def les_c4(oracle):
+        nu = 0.0045 * 32 / 180.0
+        g = oracle.Fields(oracle.D3Q27_CUDA, 192, 64, 96)
+        g.init_channel_fields(0.0045, nu, SEED, 0.1, 0.01)
+        g.init_equilibrium()
+        g.set_force((0.0045 ** 2 / 32, 0.0, 0.0))
+        return g, lambda g, n: g.step(n, 3 * nu + 0.5, use_les=True, walls=(0, 1, 0), wall_rr=1)
+
+    def cavity(q):
+        def make(oracle):
+            kind = oracle.D3Q27_CUDA if q == 27 else oracle.D3Q19
+            g = oracle.Fields(kind, 128, 128, 128)
+            g.init_equilibrium()
+            tau = 3 * 0.1 * 128 / 1000.0 + 0.5
+            return g, lambda g, n: g.step(n, tau, walls=(1, 1, 1), wall_rr=1, lid_face=3, lid_u=(0.1, 0.0, 0.0))
+        return make
+
+    def cpu_line(make, steps, what):
+        v, dt, used = oracle_mlups(make, steps)
+        return {"value": round(v, 2), "unit": "MLUPS", "cores": used, "kind": "port",
+                "sample": "%d reference-order steps of %s (FP64 oracle, C/OpenMP), %.1f s" % (steps, what, dt),
+                "cpu_model": cpu_model()}
+
+    cpu_c4 = cpu_line(les_c4, 10, "the 192x64x96 LES channel")
+    cpu_c2 = {q: cpu_line(cavity(q), 5, "the 128^3 D3Q%d cavity" % q) for q in (19, 27)}
     # C4: channel LES-WALE 192x64x96, lagged gradient (one launch per step) and reference order
     # (1.2 M cells: each step is ~30-50 us, so the steps run as graph replays)
     for st in (2, 1, 0):
@@ -257,7 +314,8 @@ def main_configs(args):
             blu = 2 * 27 * STORAGE_BYTES[st] + 6 * (8 if st == 0 else 4) + (0 if lagged else 27 * STORAGE_BYTES[st])
             out.append(line_for("MLUPS, channel LES-WALE Re_tau=180 192x64x96 D3Q27", "C4 LES %s %s" % (
                 STORAGE_NAME[st], "lagged" if lagged else "reference-order"), mlups, ms, blu, STORAGE_NAME[st],
-                args.steps, args.warmup, {"tuning": shape_of(tun), "launch": "eager" if args.no_graph else "graph"}))
+                args.steps, args.warmup, {"tuning": shape_of(tun), "launch": "eager" if args.no_graph else "graph",
+                                          "cpu_baseline": cpu_c4}))
     # C3 grid with WALE (the DNS grid, LES on): the fused RR+WALE kernel at full size
     for st in (2, 1):
         s, meta = configs.channel(storage=st, les=True)
@@ -303,7 +361,7 @@ def main_configs(args):
             log("C2 D3Q%d storage %d: %.1f MLUPS" % (q, st, mlups))
             out.append(line_for("MLUPS, lid-driven cavity 128^3 Re=1000", "C2 cavity D3Q%d %s" % (q, STORAGE_NAME[st]),
                                 mlups, ms, 2 * q * STORAGE_BYTES[st], STORAGE_NAME[st], args.steps, args.warmup,
-                                {"tuning": shape_of(tun)}))
+                                {"tuning": shape_of(tun), "cpu_baseline": cpu_c2[q]}))
     # C1: Taylor-Green 100x100x4 (40 000 cells: launch-bound), full 2 000 steps;
     # the CPU oracle runs the same 2 000 reference-order steps
     cpu = tg_cpu_baseline()
@@ -330,6 +388,24 @@ def main_configs(args):
                             {"cpu_baseline": cpu, "eager_mlups": round(mlups, 1), "graph_mlups": round(gmlups, 1)}))
     for d in out:
         print(json.dumps(d))
+
+
+def main_les_step(args):
+    """The C3-grid (384 x 128 x 192) fused steps the LES analysis compares, with
+    the library defaults and eager launches: FP16 RR+WALE (lagged), FP32
+    RR+WALE (lagged) and the FP16 RR step without LES; one JSON line each."""
+    import rrlbm
+    from rrlbm import configs
+
+    rrlbm.load()
+    for st, les in ((2, True), (1, True), (2, False)):
+        s, meta = configs.channel(storage=st, les=les)
+        mlups, ms, _ = timed(s, args.steps, args.warmup, tune=False)
+        s.close()
+        blu = 2 * 27 * STORAGE_BYTES[st] + (24 if les else 0)
+        print(json.dumps(line_for("MLUPS, channel %s Re_tau=180 384x128x192 D3Q27" % ("RR+WALE" if les else "RR"),
+                                  "C3 grid %s %s" % ("LES lagged" if les else "DNS", STORAGE_NAME[st]), mlups, ms,
+                                  blu, STORAGE_NAME[st], args.steps, args.warmup, {})), flush=True)
 
 
 def main_validate(args):
@@ -557,7 +633,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "ensemble", "configs", "validate", "copy", "turbulence"])
+                    choices=["channel", "ensemble", "configs", "validate", "copy", "turbulence", "les-step"])
     ap.add_argument("--turb-steps", type=int, default=1000000)
     ap.add_argument("--turb-window", type=int, default=300000)
     ap.add_argument("--turb-gp-steps", type=int, default=50000)
@@ -586,6 +662,8 @@ def main():
         return main_turbulence(args)
     if args.workload == "validate":
         return main_validate(args)
+    if args.workload == "les-step":
+        return main_les_step(args)
     import rrlbm
 
     rrlbm.load()

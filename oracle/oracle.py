@@ -70,6 +70,15 @@ def hermite_high(kind):
     return out.reshape(q, 10)
 
 
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops."""
+    load().orc_set_threads(ctypes.c_int(int(n)))
+
+
+def get_threads():
+    return int(load().orc_get_threads())
+
+
 def set_rr_order(order):
     """3: the reference's RR operator; 6: with its (commented) 4th-6th order terms."""
     load().orc_set_rr_order(_I(order))

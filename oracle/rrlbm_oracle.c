@@ -54,6 +54,7 @@
  *                                formulas ("parity unpinned" beyond that: no
  *                                committed reference output exists)
  */
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -182,6 +183,10 @@ typedef struct {
  * collision.cu:48-49,62-67,95-104,112-116 and BC.cu:37-38,50-54,79-88,97-101;
  * D3Q27 only). */
 static int g_rr_order = 3;
+/* Threads of the OpenMP loops (CPU timing of the oracle). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
 void orc_set_rr_order(int order) { g_rr_order = order == 6 ? 6 : 3; }
 int orc_get_rr_order(void) { return g_rr_order; }
 static int high_order(const orc_lat* L) { return g_rr_order == 6 && L->q == 27; }

@@ -25,7 +25,11 @@ HIPFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
 ]
-SOURCES = ["kernels_fused.hip", "kernels_ref.hip", "rrlbm_capi.hip"]
+# (source, object, extra flags): the fused kernels build once per storage type
+SOURCES = [("kernels_fused.hip", "kernels_fused.o", []), ("kernels_ref.hip", "kernels_ref.o", []),
+           ("rrlbm_capi.hip", "rrlbm_capi.o", [])] + [
+    ("fused_storage.hip", "fused_%s.o" % name, ["-DRRLBM_FUSED_STORAGE=%d" % k])
+    for k, name in enumerate(("f64", "f32", "f16"))]
 
 
 def _run(cmd):
@@ -38,15 +42,15 @@ def _run(cmd):
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    objs = [os.path.join(BUILD, s.replace(".hip", ".o")) for s in SOURCES]
+    objs = [os.path.join(BUILD, obj) for _, obj, _ in SOURCES]
 
-    def compile_one(pair):
-        src, obj = pair
-        _run([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+    def compile_one(item):
+        src, obj, extra = item
+        _run([HIPCC] + HIPFLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, obj)])
         return obj
 
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        list(ex.map(compile_one, zip(SOURCES, objs)))
+        list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

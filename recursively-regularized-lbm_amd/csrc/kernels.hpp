@@ -139,6 +139,25 @@ struct FusedArgs {
   PopLayout lay;        // population buffer layout
 };
 
+// Launchers of one storage type's fused kernels (fused_storage.hip, built once
+// per storage type); the storage is a tag type, one overload per translation unit.
+template <int S>
+struct StorageTag {};
+#define RRLBM_STORAGE_LAUNCHERS(S)                                                                                  \
+  hipError_t launch_fused_s(StorageTag<S>, int q, bool walls, bool les, int mode, const FusedArgs& a,               \
+                            uint32_t grid, hipStream_t s);                                                          \
+  hipError_t launch_fused_op_s(StorageTag<S>, int op, bool walls, int mode, const FusedArgs& a, uint32_t grid,      \
+                               hipStream_t s);                                                                      \
+  hipError_t launch_fused_pair_s(StorageTag<S>, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);      \
+  hipError_t launch_batched_s(StorageTag<S>, const FusedArgs* table, const uint2* work, uint32_t ntiles,            \
+                              uint32_t grid, uint32_t parity, uint32_t step, const uint32_t* step_base,             \
+                              hipStream_t s);                                                                       \
+  hipError_t launch_fdbc_rows_s(StorageTag<S>, const FusedArgs& a, hipStream_t s);
+RRLBM_STORAGE_LAUNCHERS(kF64)
+RRLBM_STORAGE_LAUNCHERS(kF32)
+RRLBM_STORAGE_LAUNCHERS(kF16)
+#undef RRLBM_STORAGE_LAUNCHERS
+
 // Host launchers (kernels_fused.hip)
 hipError_t launch_fused(int storage, int q, bool walls, bool les, int mode, const FusedArgs& a, uint32_t grid,
                         hipStream_t s);

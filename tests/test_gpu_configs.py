@@ -158,6 +158,38 @@ def test_restart_files_roundtrip(rr, tmp_path):
         assert ei.value.status == 5
 
 
+def test_restart_and_output_errors(rr, tmp_path):
+    """Restart files that end early (a truncated population file, a flow file
+    of the wrong grid) and an output directory that cannot be created return
+    RRLBM_E_IO with a message, and leave the context usable."""
+    g = random_state(oracle.D3Q27_CUDA, 12, 10, 6, seed=24, force=(1e-5, 0, 0))
+    d = str(tmp_path / "data")
+    with rr.Solver(lx=12, ly=10, lz=6, storage=0, force=(1e-5, 0, 0)) as s:
+        upload_state(s, g)
+        s.write_flow(d, 7)
+        s.write_f(d, 7)
+        fpath = os.path.join(d, "f%09d.dat" % 7)
+        raw = open(fpath, "rb").read()
+        open(fpath, "wb").write(raw[: len(raw) - 8 * 5])          # five values short
+        with pytest.raises(rr.RRLBMError) as ei:
+            s.read_restart(d, 7)
+        assert ei.value.status == 5 and "short read" in str(ei.value)
+        # a directory path below a regular file cannot be created
+        blocker = tmp_path / "plain_file"
+        blocker.write_text("x")
+        for write in (s.write_flow, s.write_f):
+            with pytest.raises(rr.RRLBMError) as ei:
+                write(str(blocker / "out"), 7)
+            assert ei.value.status == 5
+        s.write_f(d, 8)                                              # still usable
+        assert os.path.getsize(os.path.join(d, "f%09d.dat" % 8)) == 8 * 27 * g.n
+    # the step-7 flow file holds 4 x 720 values: a 12 x 10 x 7 grid needs 4 x 840
+    with rr.Solver(lx=12, ly=10, lz=7, storage=0, force=(1e-5, 0, 0)) as s4:
+        with pytest.raises(rr.RRLBMError) as ei:
+            s4.read_restart(d, 7)
+        assert ei.value.status == 5 and "short read" in str(ei.value)
+
+
 def laminar_solver(rr, storage, **kw):
     g, prm = oracle.laminar_start()
     s = rr.Solver(lx=60, ly=20, lz=2, npop=27, storage=storage, tau=prm["tau"], force=(0.0, prm["force_y"], 0.0),

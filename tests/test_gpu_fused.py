@@ -107,11 +107,35 @@ def test_fused_fp32_fp16_vs_fp64(rr, lx):
     umax = np.abs(res[0][0][1]).max()
     e32 = maxabs(res[1][0][1], res[0][0][1]) / umax
     e16 = maxabs(res[2][0][1], res[0][0][1]) / umax
+    ke = res[0][1]["kinetic_energy"]
+    k32 = abs(res[1][1]["kinetic_energy"] - ke) / ke
+    k16 = abs(res[2][1]["kinetic_energy"] - ke) / ke
+    print("FP32/FP16 vs FP64 (lx %d): u %.2e / %.2e, KE %.2e / %.2e" % (lx, e32, e16, k32, k16))
     assert e32 < 2e-5, e32
     assert e16 < 2e-2, e16
+    assert k32 < 1e-5
+    assert k16 < 2e-3
+
+
+@pytest.mark.parametrize("lagged", [0, 1])
+def test_fused_les_fp32_fp16_vs_fp64(rr, lagged):
+    """FP32 and FP16-shifted storage with WALE (lagged and reference order)
+    against the FP64 run of the same kernels on a channel state, 50 steps."""
+    g = channel_state(oracle.D3Q27_CUDA, 64, 24, 16, u_tau=0.006, visc=0.0016, seed=3, force=(1e-5, 0, 0))
+    tau = 0.5 + 3 * 0.0016
+    res = {}
+    for st in (0, 1, 2):
+        _, m, diag = run_fused(rr, g, 50, storage=st, tau=tau, force=(1e-5, 0, 0), wall_y=1, wall_rr=1, use_les=1,
+                               les_lagged=lagged)
+        res[st] = (m, diag)
+    umax = np.abs(res[0][0][1]).max()
+    e32 = maxabs(res[1][0][1], res[0][0][1]) / umax
+    e16 = maxabs(res[2][0][1], res[0][0][1]) / umax
     ke = res[0][1]["kinetic_energy"]
-    assert abs(res[1][1]["kinetic_energy"] - ke) / ke < 1e-5
-    assert abs(res[2][1]["kinetic_energy"] - ke) / ke < 2e-3
+    k32 = abs(res[1][1]["kinetic_energy"] - ke) / ke
+    k16 = abs(res[2][1]["kinetic_energy"] - ke) / ke
+    print("LES (lagged %d) FP32/FP16 vs FP64: u %.2e / %.2e, KE %.2e / %.2e" % (lagged, e32, e16, k32, k16))
+    assert e32 < 2e-5 and e16 < 2e-2 and k32 < 1e-5 and k16 < 2e-3
 
 
 def test_pack_state_error_before_pack(rr):
@@ -325,8 +349,10 @@ def test_fused_fdbc_fp32_fp16_vs_fp64(rr):
         _, m, _ = run_fused(rr, g, 50, storage=st, tau=tau, force=(1e-5, 0, 0), wall_y=1, wall_rr=2)
         res[st] = m
     umax = np.abs(res[0][1]).max()
-    assert maxabs(res[1][1], res[0][1]) / umax < 2e-5
-    assert maxabs(res[2][1], res[0][1]) / umax < 2e-2
+    e32, e16 = maxabs(res[1][1], res[0][1]) / umax, maxabs(res[2][1], res[0][1]) / umax
+    print("FDBC FP32/FP16 vs FP64: u %.2e / %.2e" % (e32, e16))
+    assert e32 < 2e-5
+    assert e16 < 2e-2
 
 
 @pytest.mark.parametrize("dims", [(10, 9, 7), (64, 5, 3)])

@@ -17,6 +17,9 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "channel": ((16, 12, 8), dict(wall_y=1, wall_rr=1), (0, 1, 0), -1),
     "channel_rows64": ((64, 6, 5), dict(wall_y=1, wall_rr=1), (0, 1, 0), -1),
+    # 73 728 cells = 288 blocks: more than a persistent grid of one workgroup
+    # per compute unit has on the 256 CUs of an MI355X, so that grid strides
+    "channel_rows128_288blocks": ((128, 24, 24), dict(wall_y=1, wall_rr=1), (0, 1, 0), -1),
     "periodic": ((10, 9, 7), dict(wall_y=0, wall_rr=0), (0, 0, 0), -1),
     "cavity": ((9, 8, 7), dict(wall_x=1, wall_y=1, wall_z=1, wall_rr=1, lid_face=3, lid_u=(0.05, 0, 0)),
                (1, 1, 1), 3),
@@ -73,15 +76,16 @@ def test_rr6_fused_matches_oracle(rr, name):
     assert maxabs(f, g3.f) > 1e-9
 
 
-@pytest.mark.parametrize("name", ["channel", "channel_rows64"])
+@pytest.mark.parametrize("name", ["channel", "channel_rows64", "channel_rows128_288blocks"])
 def test_rr6_fused_force_x_matches_oracle(rr, name):
     """The step kernel for a force along x only (the channel's), which leaves
     out the y and z force terms of the sixth-order output, against the oracle:
-    one workgroup per block and a persistent grid."""
+    one workgroup per block and a persistent grid of one workgroup per compute
+    unit (which strides over the blocks on the 288-block grid)."""
     dims, params, walls, lid = CASES[name]
     force = (3e-5, 0.0, 0.0)
     tau, steps = 0.53, 10
-    for extra in ({}, {"persistent_wg_per_cu": 2}):
+    for extra in ({}, {"persistent_wg_per_cu": 1}):
         g = random_state(oracle.D3Q27_CUDA, *dims, seed=23, amp=3e-3, umax=0.1, force=force)
         with rr.Solver(lx=dims[0], ly=dims[1], lz=dims[2], storage=0, tau=tau, force=force, rr_order=6,
                        **params, **extra) as s:
@@ -112,8 +116,10 @@ def test_rr6_fp32_fp16_vs_fp64(rr, lx):
             s.fused_unpack()
             res[st] = s.macroscopic()
     umax = np.abs(res[0][1]).max()
-    assert maxabs(res[1][1], res[0][1]) / umax < 2e-5
-    assert maxabs(res[2][1], res[0][1]) / umax < 2e-2
+    e32, e16 = maxabs(res[1][1], res[0][1]) / umax, maxabs(res[2][1], res[0][1]) / umax
+    print("RR6 FP32/FP16 vs FP64 (lx %d): u %.2e / %.2e" % (lx, e32, e16))
+    assert e32 < 2e-5
+    assert e16 < 2e-2
 
 
 def test_rr6_equilibrium_initialisation(rr):
