@@ -39,6 +39,7 @@ def test_f16_stochastic_rounding_keeps_the_laminar_acceleration(rr):
     gain = ref - u0                      # FP64 gain of U_b+ (about 1.1 over 20 000 steps)
     err_sr = np.abs(sr - ref).max() / gain[-1]
     err_rn = np.abs(rn - ref).max() / gain[-1]
+    print("FP16 eager: relative error of the U_b+ gain %.2e (stochastic), %.2e (nearest)" % (err_sr, err_rn))
     assert err_sr < 5e-3, (err_sr, sr, ref)
     # nearest rounding loses a visible share of the gain (measured about 5%)
     assert err_rn > 5 * err_sr, (err_rn, err_sr)
@@ -73,7 +74,7 @@ def test_f16_graph_replays_keep_the_laminar_acceleration(rr):
     gain = ref - u0
     err = np.abs(gr - ref).max() / gain[-1]
     print("FP16 graph replays: relative error of the U_b+ gain %.2e" % err)
-    assert err < 5e-3, (err, gr, ref)
+    assert err < 2e-3, (err, gr, ref)      # measured 6.5e-4
 
 
 def test_f16_rounding_is_reproducible(rr):

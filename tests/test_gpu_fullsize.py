@@ -66,9 +66,10 @@ def test_channel_dns_storage_precisions(rr):
     e32, e16 = maxabs(u[1][0], u[0][0]) / umax, maxabs(u[2][0], u[0][0]) / umax
     b32, b16 = (abs(u[st][1]["u_sum"][0] / u[0][1]["u_sum"][0] - 1) for st in (1, 2))
     print("channel DNS FP32/FP16 vs FP64, 100 steps: u %.2e / %.2e, U_b %.2e / %.2e" % (e32, e16, b32, b16))
-    assert e32 < 1e-5
+    # measured u 2.6e-6 / 1.3e-3, U_b 7.4e-7 / 1.1e-6 of the FP64 values
+    assert e32 < 8e-6
     assert e16 < 2e-3
-    assert b32 < 1e-5 and b16 < 1e-5
+    assert b32 < 3e-6 and b16 < 4e-6
 
 
 def test_ensemble_conservation_and_launch_forms(rr):

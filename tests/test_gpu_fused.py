@@ -111,10 +111,11 @@ def test_fused_fp32_fp16_vs_fp64(rr, lx):
     k32 = abs(res[1][1]["kinetic_energy"] - ke) / ke
     k16 = abs(res[2][1]["kinetic_energy"] - ke) / ke
     print("FP32/FP16 vs FP64 (lx %d): u %.2e / %.2e, KE %.2e / %.2e" % (lx, e32, e16, k32, k16))
-    assert e32 < 2e-5, e32
-    assert e16 < 2e-2, e16
-    assert k32 < 1e-5
-    assert k16 < 2e-3
+    # about 3x the measured errors on MI355X (u 1.4e-6 / 9.9e-4, KE 1.2e-6 / 1.6e-5)
+    assert e32 < 5e-6, e32
+    assert e16 < 3e-3, e16
+    assert k32 < 4e-6, k32
+    assert k16 < 5e-5, k16
 
 
 @pytest.mark.parametrize("lagged", [0, 1])
@@ -135,7 +136,8 @@ def test_fused_les_fp32_fp16_vs_fp64(rr, lagged):
     k32 = abs(res[1][1]["kinetic_energy"] - ke) / ke
     k16 = abs(res[2][1]["kinetic_energy"] - ke) / ke
     print("LES (lagged %d) FP32/FP16 vs FP64: u %.2e / %.2e, KE %.2e / %.2e" % (lagged, e32, e16, k32, k16))
-    assert e32 < 2e-5 and e16 < 2e-2 and k32 < 1e-5 and k16 < 2e-3
+    # about 3x the measured errors (u 1.4e-6 / 8.6e-4, KE 1.2e-6 / 1.2e-5)
+    assert e32 < 5e-6 and e16 < 3e-3 and k32 < 4e-6 and k16 < 4e-5, (e32, e16, k32, k16)
 
 
 def test_pack_state_error_before_pack(rr):
@@ -351,8 +353,8 @@ def test_fused_fdbc_fp32_fp16_vs_fp64(rr):
     umax = np.abs(res[0][1]).max()
     e32, e16 = maxabs(res[1][1], res[0][1]) / umax, maxabs(res[2][1], res[0][1]) / umax
     print("FDBC FP32/FP16 vs FP64: u %.2e / %.2e" % (e32, e16))
-    assert e32 < 2e-5
-    assert e16 < 2e-2
+    assert e32 < 5e-6      # measured 1.3e-6
+    assert e16 < 3e-3      # measured 8.7e-4
 
 
 @pytest.mark.parametrize("dims", [(10, 9, 7), (64, 5, 3)])

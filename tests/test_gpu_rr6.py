@@ -118,8 +118,8 @@ def test_rr6_fp32_fp16_vs_fp64(rr, lx):
     umax = np.abs(res[0][1]).max()
     e32, e16 = maxabs(res[1][1], res[0][1]) / umax, maxabs(res[2][1], res[0][1]) / umax
     print("RR6 FP32/FP16 vs FP64 (lx %d): u %.2e / %.2e" % (lx, e32, e16))
-    assert e32 < 2e-5
-    assert e16 < 2e-2
+    assert e32 < 5e-6      # measured 1.4e-6
+    assert e16 < 3e-3      # measured 7.6e-4 .. 9.1e-4
 
 
 def test_rr6_equilibrium_initialisation(rr):
