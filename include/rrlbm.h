@@ -107,7 +107,7 @@ typedef struct rrlbm_params {
                                    0 selects 4096 */
     int32_t xcd_remap;          /* order of the blocks of cells over the workgroup IDs, which the
                                    dispatcher deals round-robin to the 8 XCDs: 0 linear;
-                                   1 one contiguous slab per XCD; C = 2^k (2..64) runs of C
+                                   1 one contiguous slab per XCD; C (2..65536) runs of C
                                    consecutive blocks per XCD; -1 = library default (32 on grids
                                    of at least 2048 blocks of 256 cells, else 0) */
     int32_t cells_per_thread;   /* fused D3Q27 step without LES: 1 or 2 cells per thread

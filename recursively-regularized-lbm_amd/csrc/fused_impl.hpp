@@ -242,13 +242,23 @@ RRLBM_INLINE __device__ void st_v(void* u, uint32_t cell, C x, C y, C z) {
   reinterpret_cast<Vel3<C>*>(u)[cell] = Vel3<C>{x, y, z};
 }
 
-// Derivative along one axis (LES.cu:19-33): central difference, or at a wall
-// cell the one-sided form with the half-way wall velocity taken as zero.
+// Twice the velocity derivative along one axis (LES.cu:19-33): the undivided
+// central difference v(+1) - v(-1), or at a wall cell twice the one-sided form
+// with the half-way wall velocity taken as zero, (3 v0 + v(+1)) / 3 at the low
+// wall and -(3 v0 + v(-1)) / 3 at the high one.
 template <typename C>
-RRLBM_INLINE __device__ C ddir(C v0, C vm, C vp, bool lo, bool hi) {
-  if (lo) return (C(3) * v0 + vp) * C(1.0 / 3.0);
-  if (hi) return (C(-3) * v0 - vm) * C(1.0 / 3.0);
-  return C(0.5) * (vp - vm);
+RRLBM_INLINE __device__ Vel3<C> d2_central(const Vel3<C>& vm, const Vel3<C>& vp) {
+  return Vel3<C>{vp.x - vm.x, vp.y - vm.y, vp.z - vm.z};
+}
+template <typename C>
+RRLBM_INLINE __device__ Vel3<C> d2_low_wall(const Vel3<C>& v0, const Vel3<C>& vp) {
+  const C k = C(2.0 / 3.0);
+  return Vel3<C>{(C(3) * v0.x + vp.x) * k, (C(3) * v0.y + vp.y) * k, (C(3) * v0.z + vp.z) * k};
+}
+template <typename C>
+RRLBM_INLINE __device__ Vel3<C> d2_high_wall(const Vel3<C>& v0, const Vel3<C>& vm) {
+  const C k = C(2.0 / 3.0);
+  return Vel3<C>{(C(-3) * v0.x - vm.x) * k, (C(-3) * v0.y - vm.y) * k, (C(-3) * v0.z - vm.z) * k};
 }
 
 // Byte offset (without the population term) of the first cell of row
@@ -319,6 +329,64 @@ RRLBM_INLINE __device__ void gp_force(const FusedArgs& a, int k, uint32_t x, uin
     fy = -f0 * Amp0 * C(0.5) * LYq * cy * cos(beta9 * pi2 * xph) * cz;
     fz = -f0 * Amp0 * C(0.5) * LZ / gamma9 * sy * cos(beta9 * pi2 * xph) * sin(gamma9 * pi2 * izc / LZ);
   }
+}
+
+// The WALE stencil of one cell: the velocity field a.u_in (previous step) at
+// the six face neighbours (periodic wrap).
+template <typename C>
+struct LesStencil {
+  Vel3<C> xm, xp, ym, yp, zm, zp;
+};
+
+// Load the stencil of cell (x, y, z); xs, ys, zs: the periodic neighbour
+// coordinates {+1, 0, -1}.  With ROWS the five stencil rows are
+// wavefront-uniform (scalar bases) and the lane adds its x offset.
+template <typename C>
+RRLBM_INLINE __device__ LesStencil<C> les_load(const FusedArgs& a, uint32_t x, uint32_t y, uint32_t z,
+                                               const uint32_t (&xs)[3], const uint32_t (&ys)[3],
+                                               const uint32_t (&zs)[3]) {
+  constexpr uint32_t vb = sizeof(Vel3<C>);
+  auto vrow = [&](uint32_t yy, uint32_t zz) {
+    return reinterpret_cast<const char*>(a.u_in) + (uint64_t)((zz * a.ly + yy) * a.lx) * vb;
+  };
+  auto vat = [](const char* row, uint32_t off) { return *reinterpret_cast<const Vel3<C>*>(row + off); };
+  const char* r00 = vrow(y, z);
+  const uint32_t o0 = x * vb;
+  LesStencil<C> st;
+  st.xm = vat(r00, xs[2] * vb);
+  st.xp = vat(r00, xs[0] * vb);
+  st.ym = vat(vrow(ys[2], z), o0);
+  st.yp = vat(vrow(ys[0], z), o0);
+  st.zm = vat(vrow(y, zs[2]), o0);
+  st.zp = vat(vrow(y, zs[0]), o0);
+  return st;
+}
+
+// LES relaxation factor 1 - 1/(tau + tau_t) of cell (x, y, z) from its WALE
+// stencil (LES.cu:4-98); wall cells load their own velocity here for the
+// one-sided forms.
+template <typename C>
+RRLBM_INLINE __device__ C les_omega_of(const FusedArgs& a, const LesStencil<C>& st, uint32_t x, uint32_t y,
+                                       uint32_t z) {
+  const bool wy = (a.walls & 2) != 0, wx = (a.walls & 1) != 0, wz = (a.walls & 4) != 0;
+  const bool xlo = wx && x == 0, xhi = wx && x == a.lx - 1;
+  const bool ylo = wy && y == 0, yhi = wy && y == a.ly - 1;
+  const bool zlo = wz && z == 0, zhi = wz && z == a.lz - 1;
+  // D = 2 grad u; rows: Dx = d/dx of (u, v, w), ...
+  Vel3<C> Dx = d2_central(st.xm, st.xp), Dy = d2_central(st.ym, st.yp), Dz = d2_central(st.zm, st.zp);
+  // wall cells (with ROWS the y and z tests are uniform per wavefront):
+  // the one-sided forms, which need the cell's own velocity
+  if (xlo || xhi || ylo || yhi || zlo || zhi) {
+    const Vel3<C> v0 = reinterpret_cast<const Vel3<C>*>(a.u_in)[(z * a.ly + y) * a.lx + x];
+    if (xlo) Dx = d2_low_wall(v0, st.xp);
+    else if (xhi) Dx = d2_high_wall(v0, st.xm);
+    if (ylo) Dy = d2_low_wall(v0, st.yp);
+    else if (yhi) Dy = d2_high_wall(v0, st.ym);
+    if (zlo) Dz = d2_low_wall(v0, st.zp);
+    else if (zhi) Dz = d2_high_wall(v0, st.zm);
+  }
+  const C tau_eff = C(a.tau) + wale_taut_d2<C>(Dx.x, Dy.x, Dz.x, Dx.y, Dy.y, Dz.y, Dx.z, Dy.z, Dz.z, C(a.cw));
+  return C(1) - recip(tau_eff);
 }
 
 // FDBC at a y-wall cell (BC.cu:129-309, the drop-in k_fdbc_* in one place):
@@ -452,6 +520,12 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   const char* src = reinterpret_cast<const char*>(a.src);
   char* dst = reinterpret_cast<char*>(a.dst);
 
+  // LES step: the WALE stencil's velocity loads go out before the population
+  // gather, and the eddy viscosity is computed right after it, so that the
+  // arithmetic runs while the population loads are still in flight (loads
+  // complete in order)
+  [[maybe_unused]] LesStencil<C> les_st;
+  if constexpr (LES && MODE == kStep) les_st = les_load<C>(a, x, y, z, xs, ys, zs);
   // FP16 D3Q27 collisions that need only the moments (not the populations
   // after the wall treatment) keep the gathered FP16 values and take their
   // moments with moments_q27_f16; wall cells convert them first.
@@ -472,6 +546,8 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   } else {
     gather_lane<S, L>(a, x, y, z, h);
   }
+  [[maybe_unused]] C les_om;
+  if constexpr (LES && MODE == kStep) les_om = les_omega_of<C>(a, les_st, x, y, z);
   // this cell in the source and destination buffers: base (scalar with ROWS) + offset
   const char* own_src = src;
   char* own_dst = dst;
@@ -586,34 +662,12 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   } else {
     C omega;
     if constexpr (LES) {
-      const bool wy = (a.walls & 2) != 0, wx = (a.walls & 1) != 0, wz = (a.walls & 4) != 0;
-      const bool xlo = wx && x == 0, xhi = wx && x == a.lx - 1;
-      const bool ylo = wy && y == 0, yhi = wy && y == a.ly - 1;
-      const bool zlo = wz && z == 0, zhi = wz && z == a.lz - 1;
-      // the five velocity rows of the stencil (scalar with ROWS) and the
-      // lane's x offsets; xs[0] is x+1 and xs[2] is x-1
+      omega = les_om;
       constexpr uint32_t vb = sizeof(Vel3<C>);
+      const uint32_t o0 = x * vb;
       auto vrow = [&](const void* base, uint32_t yy, uint32_t zz) {
         return reinterpret_cast<const char*>(base) + (uint64_t)((zz * a.ly + yy) * a.lx) * vb;
       };
-      auto vat = [](const char* row, uint32_t off) { return *reinterpret_cast<const Vel3<C>*>(row + off); };
-      const char* r00 = vrow(a.u_in, y, z);
-      const uint32_t o0 = x * vb, oxp = xs[0] * vb, oxm = xs[2] * vb;
-      const Vel3<C> v0 = vat(r00, o0);
-      const Vel3<C> vxm = vat(r00, oxm), vxp = vat(r00, oxp);
-      const Vel3<C> vym = vat(vrow(a.u_in, ys[2], z), o0), vyp = vat(vrow(a.u_in, ys[0], z), o0);
-      const Vel3<C> vzm = vat(vrow(a.u_in, y, zs[2]), o0), vzp = vat(vrow(a.u_in, y, zs[0]), o0);
-      const C dudx = ddir<C>(v0.x, vxm.x, vxp.x, xlo, xhi);
-      const C dvdx = ddir<C>(v0.y, vxm.y, vxp.y, xlo, xhi);
-      const C dwdx = ddir<C>(v0.z, vxm.z, vxp.z, xlo, xhi);
-      const C dudy = ddir<C>(v0.x, vym.x, vyp.x, ylo, yhi);
-      const C dvdy = ddir<C>(v0.y, vym.y, vyp.y, ylo, yhi);
-      const C dwdy = ddir<C>(v0.z, vym.z, vyp.z, ylo, yhi);
-      const C dudz = ddir<C>(v0.x, vzm.x, vzp.x, zlo, zhi);
-      const C dvdz = ddir<C>(v0.y, vzm.y, vzp.y, zlo, zhi);
-      const C dwdz = ddir<C>(v0.z, vzm.z, vzp.z, zlo, zhi);
-      const C tau_eff = C(a.tau) + wale_taut<C>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, C(a.cw));
-      omega = C(1) - recip(tau_eff);
       if (a.write_u && active) {
         uint32_t ow = o0;
         asm volatile("" : "+v"(ow));  // 32-bit lane offset in this block (scalar-base store)
@@ -666,14 +720,14 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
 // Block t of the launch's tile sequence -> block of cells.  The dispatcher
 // deals consecutive workgroup IDs round-robin to the 8 XCDs (each with its own
 // L2).  xcd_remap 0: linear; 1: each XCD sweeps one contiguous 1/8 of the
-// domain; C = 2^k > 1: each XCD takes runs of C consecutive blocks, so the
+// domain; C > 1: each XCD takes runs of C consecutive blocks, so the
 // cache lines two neighbouring blocks share (the +-1 x shift of the gather)
 // mostly stay in one L2 while all XCDs still work on nearby memory.
 RRLBM_INLINE __device__ uint32_t block_of(const FusedArgs& a, uint32_t t) {
   if (a.xcd_remap == 0) return t;
   if (a.xcd_remap == 1) return (t & 7u) * a.blocks_per_xcd + (t >> 3);
-  const uint32_t j = t >> 3, c = a.xcd_remap;
-  return ((j / c) * 8u + (t & 7u)) * c + (j & (c - 1u));
+  const uint32_t j = t >> 3, c = a.xcd_remap, q = fastdiv(j, a.div_remap);
+  return (q * 8u + (t & 7u)) * c + (j - q * c);
 }
 
 // One workgroup per block of 256 cells (grid = a.tiles).

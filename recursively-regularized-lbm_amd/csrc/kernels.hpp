@@ -133,7 +133,8 @@ struct FusedArgs {
   uint32_t sr_salt;     // per-instance offset of the dither (batched ensembles)
   uint32_t nblocks;     // workgroups covering the domain
   uint32_t blocks_per_xcd;
-  uint32_t xcd_remap;   // 0 linear block order; 1 one contiguous slab per XCD; C = 2^k: runs of C blocks per XCD
+  uint32_t xcd_remap;   // 0 linear block order; 1 one contiguous slab per XCD; C > 1: runs of C blocks per XCD
+  FastDiv div_remap;    // division by xcd_remap (C > 1)
   uint32_t tiles;       // length of the block sequence (nblocks rounded up for the remapped orders);
                         // the grid may be smaller (persistent workgroups stride over it)
   PopLayout lay;        // population buffer layout

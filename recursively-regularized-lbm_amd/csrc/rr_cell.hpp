@@ -775,8 +775,14 @@ RRLBM_HD RRLBM_INLINE float wale_sqrt(float x) {
 // Sd = M - (tr M / 3) I with M = S.S + w w^T, and S:S = tr(S.S).  The powers
 // use x^1.5 = x sqrt(x), x^2.5 = x^2 sqrt(x), x^1.25 = x sqrt(sqrt(x)); a
 // vanishing gradient (0/0 in the reference) gives tau_t = 0.
+//
+// The _d2 forms take D = 2 g, the undivided central differences
+// v(x+1) - v(x-1): tau_t is homogeneous of degree 1 in g (numerator degree 6,
+// denominator degree 5), so tau_t(g) = tau_t(D) / 2, and the factor 1/2 goes
+// into the constant.  Scaling by 2 is exact in binary floating point (and so
+// are the square roots of 4x and 16x), so both forms give the same bits.
 template <typename C>
-RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz, C dwdx, C dwdy, C dwdz, C cw) {
+RRLBM_HD RRLBM_INLINE C wale_taut_d2(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz, C dwdx, C dwdy, C dwdz, C cw) {
   const C h = C(0.5);
   // S (diagonal a, off-diagonal b) and w
   const C a1 = dudx, a2 = dvdy, a3 = dwdz;
@@ -795,8 +801,15 @@ RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz
   const C sd = d1 * d1 + d2 * d2 + d3 * d3 + C(2) * (m12 * m12 + m13 * m13 + m23 * m23);  // Sd:Sd
   const C rs = wale_sqrt(ss), rd = wale_sqrt(sd);
   const C den = ss * ss * rs + sd * wale_sqrt(rd);
-  const C num = C(3) * cw * cw * sd * rd;
+  const C num = C(1.5) * cw * cw * sd * rd;
   return den > C(0) ? num * recip(den) : C(0);
+}
+
+// The same from the velocity gradient itself (g_ij = du_i/dx_j).
+template <typename C>
+RRLBM_HD RRLBM_INLINE C wale_taut(C dudx, C dudy, C dudz, C dvdx, C dvdy, C dvdz, C dwdx, C dwdy, C dwdz, C cw) {
+  const C t = C(2);
+  return wale_taut_d2<C>(t * dudx, t * dudy, t * dudz, t * dvdx, t * dvdy, t * dvdz, t * dwdx, t * dwdy, t * dwdz, cw);
 }
 
 }  // namespace rrlbm

@@ -142,6 +142,7 @@ void set_blocks(FusedArgs& a, uint32_t nblocks, int remap) {
   a.nblocks = nblocks;
   a.xcd_remap = (uint32_t)remap;
   a.blocks_per_xcd = (nblocks + 7) / 8;
+  a.div_remap = make_fastdiv(remap > 1 ? (uint32_t)remap : 1u);
   if (remap == 0)
     a.tiles = nblocks;
   else if (remap == 1)
@@ -390,7 +391,7 @@ int read_fields(rrlbm_ctx* c, const char* path, double* const* dev, int nf, size
 
 extern "C" {
 
-const char* rrlbm_version(void) { return "rrlbm-gfx950 1.0 (abi 1)"; }
+const char* rrlbm_version(void) { return "rrlbm-gfx950 1.1 (abi 1)"; }
 
 int rrlbm_params_default(rrlbm_params* p) {
   if (!p) return fail(nullptr, RRLBM_E_INVALID, "null params");
@@ -460,8 +461,8 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     if (!walled) return fail(nullptr, RRLBM_E_INVALID, "lid_face %d lies on a periodic axis", p.lid_face);
   }
   if (p.f16_scale == 0.0) p.f16_scale = 4096.0;
-  if (p.xcd_remap < -1 || p.xcd_remap > 64 || (p.xcd_remap > 1 && (p.xcd_remap & (p.xcd_remap - 1))))
-    return fail(nullptr, RRLBM_E_INVALID, "xcd_remap must be -1, 0, 1 or a power of two up to 64");
+  if (p.xcd_remap < -1 || p.xcd_remap > 65536)
+    return fail(nullptr, RRLBM_E_INVALID, "xcd_remap must be -1, 0, 1 or a run length up to 65536");
   // default block order: runs of 32 blocks per XCD keep the blocks that share
   // cache lines in one L2 (measured: FP16 channel reads 1.06x instead of 1.23x
   // the algorithmic bytes); small grids keep the linear order

@@ -68,7 +68,7 @@ def test_invalid_parameters_are_reported_not_fatal(rr):
     # limits and option combinations, with the exact status: 1 invalid, 6 unsupported
     cases = [(dict(lx=2048, ly=1024, lz=1024), 1),                      # 2^31 cells
              (dict(lx=1024, ly=1024, lz=1000, storage=0), 1),           # a population array over 4 GiB
-             (dict(xcd_remap=3), 1), (dict(cells_per_thread=3), 1), (dict(persistent_wg_per_cu=65), 1),
+             (dict(xcd_remap=-2), 1), (dict(xcd_remap=65537), 1), (dict(cells_per_thread=3), 1), (dict(persistent_wg_per_cu=65), 1),
              (dict(collision_model=2), 1), (dict(collision_model=1, wall_rr=1), 6), (dict(collision_model=1, npop=19), 6),
              (dict(rr_order=5), 1), (dict(rr_order=6, npop=19), 6), (dict(rr_order=6, use_les=1), 6),
              (dict(wall_rr=3), 1), (dict(wall_rr=2, wall_y=1, wall_x=1), 6), (dict(wall_rr=2, wall_y=1, ly=3), 6),

@@ -193,8 +193,8 @@ def test_layout_parameters_are_reported_and_checked(rr):
     with rr.Solver(lx=100, ly=10, lz=4, storage=2) as s:
         p = s.params
         assert p.pop_group_rows == -1 and p.pop_pad_bytes == 2048
-    for bad in (dict(pop_group_rows=-2), dict(pop_pad_bytes=24), dict(pop_pad_bytes=2 << 20), dict(xcd_remap=3),
-                dict(xcd_remap=128), dict(cells_per_thread=3), dict(persistent_wg_per_cu=65)):
+    for bad in (dict(pop_group_rows=-2), dict(pop_pad_bytes=24), dict(pop_pad_bytes=2 << 20), dict(xcd_remap=-2),
+                dict(xcd_remap=65537), dict(cells_per_thread=3), dict(persistent_wg_per_cu=65)):
         with pytest.raises(rr.RRLBMError) as ei:
             rr.Solver(lx=8, ly=8, lz=8, **bad)
         assert ei.value.status == 1
@@ -209,6 +209,7 @@ PERSIST_DIMS = (128, 64, 40)
 @pytest.mark.parametrize("shape", [dict(persistent_wg_per_cu=1), dict(persistent_wg_per_cu=3, xcd_remap=1),
                                    dict(cells_per_thread=2, persistent_wg_per_cu=1), dict(xcd_remap=1),
                                    dict(xcd_remap=4), dict(xcd_remap=16, persistent_wg_per_cu=1),
+                                   dict(xcd_remap=3), dict(xcd_remap=24, persistent_wg_per_cu=1),
                                    dict(cells_per_thread=2, xcd_remap=2)])
 def test_launch_shapes_are_bit_identical(rr, shape):
     """Persistent grids (workgroups striding over the blocks; the unpack and
