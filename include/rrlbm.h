@@ -280,7 +280,7 @@ typedef struct rrlbm_tuning {
 } rrlbm_tuning;
 /* Times ctx's fused step over candidate population layouts (rows per group,
  * padding) and launch shapes (1 or 2 cells per thread, block order over the
- * XCDs: linear, slabs, runs of 4 or 32 blocks; one workgroup per block or a
+ * XCDs: linear, slabs, runs of 4, 16, 32 or 64 blocks; one workgroup per block or a
  * persistent grid of 4 or 8 per compute unit), best of three interleaved
  * rounds of trial_steps steps each; then times the fastest against the
  * library default in rounds of 3 * trial_steps and keeps the default unless

@@ -1231,7 +1231,7 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   // two cells per thread only where use_pair() would launch the pair kernel
   const bool pair_ok = c->q == 27 && !c->p.use_les && !is_bgk(c->p) && !is_rr6(c->p) && c->p.wall_rr != 2;
   for (int cpt : {1, 2})
-    for (int remap : {0, 1, 4, 32})
+    for (int remap : {0, 1, 4, 16, 32, 64})
       for (int pers : {0, 4, 8}) {
         if (cpt == 2 && !pair_ok) continue;
         shapes.push_back({base.group_rows, base.pad, cpt, remap, pers});
