@@ -140,8 +140,10 @@ bool is_rr6(const rrlbm_params& p) { return p.collision_model == RRLBM_COLLISION
 
 void set_blocks(FusedArgs& a, uint32_t nblocks, int remap) {
   a.nblocks = nblocks;
-  a.xcd_remap = (uint32_t)remap;
   a.blocks_per_xcd = (nblocks + 7) / 8;
+  // a run at least as long as one XCD's share is one contiguous slab per XCD
+  if (remap > 1 && (uint32_t)remap >= a.blocks_per_xcd) remap = 1;
+  a.xcd_remap = (uint32_t)remap;
   a.div_remap = make_fastdiv(remap > 1 ? (uint32_t)remap : 1u);
   if (remap == 0)
     a.tiles = nblocks;

@@ -94,6 +94,35 @@ def test_fused_les(rr, lagged, lx):
         assert err < 1e-5  # lag of one step in tau_t only
 
 
+@pytest.mark.parametrize("lx", [40, 64])
+def test_fused_les_walls_on_every_axis(rr, lx):
+    """WALE's one-sided derivatives at walls on all three axes (with rows of 64
+    cells the x walls are per lane, the y and z walls uniform per wavefront):
+    the fused FP64 step in the reference order (velocity pass, then the step)
+    equals the drop-in kernels' step (LES_WALE, collision_RR, streaming, BB,
+    macrovar, RRBC, macrovar_BN) to 1e-12.  The drop-in LES_WALE applies the
+    reference's y-wall form (LES.cu:19-33, pinned against the oracle) to each
+    walled axis."""
+    g = random_state(oracle.D3Q27_CUDA, lx, 12, 10, seed=31, umax=0.04, force=(1e-5, 0, 0))
+    kw = dict(lx=lx, ly=12, lz=10, npop=27, storage=0, tau=0.51, force=(1e-5, 0, 0), wall_x=1, wall_y=1, wall_z=1,
+              wall_rr=1, lid_face=3, lid_u=(0.05, 0, 0), use_les=1, les_lagged=0)
+    steps = 5
+    with rr.Solver(**kw) as s:
+        upload_state(s, g)
+        s.step_reference(steps)
+        ref_f, ref_m = s.download(rr.Field.F), s.macroscopic()
+    with rr.Solver(**kw) as s:
+        upload_state(s, g)
+        s.fused_pack()
+        s.fused_step(steps - 1)
+        s.fused_unpack()
+        f, m = s.download(rr.Field.F), s.macroscopic()
+    ef = maxabs(f, ref_f)
+    em = max(maxabs(a, b) for a, b in zip(m, ref_m))
+    assert np.abs(ref_m[1]).max() > 1e-3  # a flow with gradients at every wall
+    assert ef < 1e-12 and em < 1e-12, (ef, em)
+
+
 @pytest.mark.parametrize("lx", [32, 64])
 def test_fused_fp32_fp16_vs_fp64(rr, lx):
     """FP32 and FP16-shifted storage (FP32 arithmetic) against the FP64 run of
