@@ -42,9 +42,12 @@ STORAGE_BYTES = {0: 8, 1: 4, 2: 2}
 STORAGE_NAME = {0: "f64", 1: "f32", 2: "f16"}
 
 
-def run_variant(rrlbm, storage, steps, warmup, tune=True):
+def run_variant(rrlbm, storage, steps, warmup, tune=True, les=False):
+    """One storage variant of the channel step on the full grid (les: RR+WALE,
+    the one-launch lagged-gradient step); MLUPS over the wall clock, the
+    fused kernel's mean duration from HIP events on the library's stream."""
     s = rrlbm.Solver(lx=LX, ly=LY, lz=LZ, npop=27, storage=storage, tau=TAU, force=(FORCE, 0.0, 0.0),
-                     wall_y=1, wall_rr=1, use_les=0)
+                     wall_y=1, wall_rr=1, use_les=int(les), cw=0.5)
     try:
         s.init_channel(U_TAU, VISC, SEED, 0.1, 0.01)
         tuning = s.autotune(trial_steps=10) if tune else None
@@ -61,7 +64,8 @@ def run_variant(rrlbm, storage, steps, warmup, tune=True):
     n = LX * LY * LZ
     mlups = n * steps / wall / 1e6
     per_launch = kernel_ms / 1e3 / steps
-    blu = 2 * 27 * STORAGE_BYTES[storage]
+    # LES: + the velocity field (3 components in the compute type) read once and written once
+    blu = 2 * 27 * STORAGE_BYTES[storage] + (6 * (8 if storage == 0 else 4) if les else 0)
     achieved = blu * n / per_launch / 1e9
     return {
         "mlups": mlups,
@@ -629,6 +633,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-les", action="store_true", help="channel workload: leave out the RR+WALE variants")
     ap.add_argument("--no-tune", action="store_true", help="library default layout and launch shape (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
@@ -669,6 +674,8 @@ def main():
     rrlbm.load()
     variants = [int(v) for v in args.variants.split(",")]
     res = {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune) for st in variants}
+    les = {} if args.no_les else {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune, les=True)
+                                  for st in (1, 2)}
     head = res[variants[0]]
     st0 = variants[0]
     line = {
@@ -700,6 +707,12 @@ def main():
                                                                          "cells_per_thread", "xcd_remap",
                                                                          "persistent_wg_per_cu")}}
                      for st, r in res.items()},
+        # the same grid with WALE LES (lagged gradient, one launch per step)
+        "variants_les": {STORAGE_NAME[st]: {"mlups": round(r["mlups"], 1), "frac": round(r["frac"], 4),
+                                             "achieved_gbs": round(r["achieved_gbs"], 1),
+                                             "bytes_per_lu": r["bytes_per_lu"], "kernel_ms": round(r["kernel_ms"], 5),
+                                             "tuning": shape_of(r["tuning"])}
+                         for st, r in les.items()},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     print(json.dumps(line))

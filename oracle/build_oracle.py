@@ -7,6 +7,10 @@
                        the committed D3Q27 output was produced), run in _ref/
                        to produce D3Q19 output files.  Nothing from the
                        reference is copied into the repository.
+  _ref/tg27/        -- (--tg27) the reference's D3Q27 Taylor-Green program,
+                       built and run the same way; its output equals the
+                       committed RRBMa0.01L100.txt column for column and its
+                       first line times the reference's CPU path here.
   _ref/cavity/      -- the reference's lid-driven cavity solver (Cavity/*.f90,
                        128x128x2, D3Q27 RR, Re 1000, lid x = LX moving in +y)
                        compiled unmodified from its sources and run until its
@@ -56,6 +60,46 @@ def build_reference_tg19(run=True):
     if run and not os.path.exists(os.path.join(REF_DIR, "D3Q19RR33BMa0.01L50.txt")):
         subprocess.run([exe], check=True, cwd=REF_DIR, stdout=subprocess.DEVNULL)
     return REF_DIR
+
+
+def build_reference_tg27(run=True):
+    """Compile Taylor_Green/D3Q27RR.f90 into _ref/tg27/ (amdflang -O2, default
+    REAL kind 4) and (optionally) run it there: 20 000 steps of the 100 x 100 x 2
+    Taylor-Green vortex, serial, a few minutes; it writes RRBMa0.01L100*.txt
+    with its own execution time on the first line.  This build reproduces the
+    committed Taylor_Green/RRBMa0.01L100.txt column for column, which also
+    times the reference's own CPU path on this machine.
+
+    Returns the output file, or None when the reference or amdflang is absent."""
+    src = os.path.join(REFERENCE, "Taylor_Green", "D3Q27RR.f90")
+    if not (os.path.exists(src) and os.path.exists(AMDFLANG)):
+        return None
+    work = os.path.join(REF_DIR, "tg27")
+    os.makedirs(work, exist_ok=True)
+    exe = os.path.join(work, "tg_d3q27")
+    if not os.path.exists(exe):
+        subprocess.run([AMDFLANG, "-O2", "-o", exe, src], check=True, cwd=work)
+    out = os.path.join(work, "RRBMa0.01L100.txt")
+    if run and not os.path.exists(out):
+        subprocess.run([exe], check=True, cwd=work, stdout=subprocess.DEVNULL)
+    return out
+
+
+def same_midline(path_a, path_b):
+    """Largest difference between two RRBMa0.01L100.txt midline tables."""
+    import numpy as np
+
+    def rows(p):
+        out = []
+        with open(p) as fh:
+            for line in fh:
+                t = line.split()
+                if t and not t[0].startswith("#") and t[0] != "Execution":
+                    out.append([float(v) for v in t])
+        return np.array(out)
+
+    a, b = rows(path_a), rows(path_b)
+    return float(np.abs(a - b).max()) if a.shape == b.shape else float("inf")
 
 
 CAVITY_SOURCES = ["var_inc.f90", "para.f90", "RR.f90", "collision.f90", "initial.f90", "streaming.f90", "main.f90"]
@@ -117,3 +161,10 @@ if __name__ == "__main__":
         print("reference build:", build_reference_tg19())
     if "--cavity" in sys.argv:
         print("reference cavity dump:", build_reference_cavity())
+    if "--tg27" in sys.argv:
+        out = build_reference_tg27()
+        if out:
+            with open(out) as fh:
+                print(fh.readline().strip())
+            print("largest difference from the committed output:",
+                  same_midline(out, os.path.join(REFERENCE, "Taylor_Green", "RRBMa0.01L100.txt")))
