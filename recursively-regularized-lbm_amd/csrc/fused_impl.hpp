@@ -70,12 +70,16 @@ RRLBM_INLINE __device__ uint32_t cell_off(const PopLayout& l, uint32_t c) {
   return pop_cell_off(l, c, (uint32_t)sizeof(typename StoreT<S>::T));
 }
 
+// T in the global address space (device memory): an address formed from an
+// integer keeps its global load / store instructions.
+template <class T>
+using GlobalT = __attribute__((address_space(1))) T;
+
 // Post-collision populations are written once and read by the next launch:
 // non-temporal stores keep them from displacing the gather's cache lines.
 template <class T>
-RRLBM_INLINE __device__ void store_raw(void* base, uint64_t seg, int i, uint32_t off, T v) {
-  char* p = reinterpret_cast<char*>(base) + (uint64_t)i * seg;
-  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p + off));
+RRLBM_INLINE __device__ void store_raw(uint64_t base, uint32_t off, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<GlobalT<T>*>(base + off));
 }
 
 // Stochastic rounding of an FP32 value to FP16 with gfx950's v_cvt_sr_f16_f32:
@@ -114,7 +118,7 @@ RRLBM_INLINE __device__ uint32_t dither_base(uint32_t cell, uint32_t step, uint3
 }
 
 // Store the Q post-collision values of one cell (FP16: scaled values) at
-// base + i*seg + off.
+// base + i*seg + off; base is uniform across the wavefront.
 template <int S, int Q>
 RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t off, uint32_t cell,
                                         const typename StoreT<S>::C (&out)[Q]) {
@@ -122,7 +126,14 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
   // Each branch takes a fresh 32-bit copy of the lane offset: the offset is
   // also used by the wall path, and a 64-bit extension of it shared across
   // blocks turns every store into a per-lane 64-bit address add instead of a
-  // scalar-base + 32-bit-offset store.
+  // scalar-base + 32-bit-offset store.  The population base advances by seg
+  // in a scalar register pair held opaque, as in the gather.
+  uint64_t pb = reinterpret_cast<uint64_t>(base);
+  auto put = [&](T v) {
+    store_raw<T>(pb, off, v);
+    pb += a.lay.seg;
+    asm volatile("" : "+s"(pb));
+  };
   if constexpr (S == kF16) {
     if (a.f16_round == 0) {
       asm volatile("" : "+v"(off));
@@ -133,7 +144,7 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
       rr_half2 z = {_Float16(0), _Float16(0)};
       static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        store_raw<T>(base, a.lay.seg, i, off, f16_stochastic(z, out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
+        put(f16_stochastic(z, out[i], r0 + uint32_t(i) * 0xC2B2AE3Du));
       });
       return;
     }
@@ -141,7 +152,7 @@ RRLBM_INLINE __device__ void store_cell(const FusedArgs& a, void* base, uint32_t
   asm volatile("" : "+v"(off));
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    store_raw<T>(base, a.lay.seg, i, off, T(out[i]));
+    put(T(out[i]));
   });
 }
 
@@ -266,6 +277,14 @@ RRLBM_INLINE __device__ Vel3<C> d2_high_wall(const Vel3<C>& v0, const Vel3<C>& v
 RRLBM_INLINE __device__ uint64_t row_offset(const FusedArgs& a, uint32_t row_b, uint32_t yy, uint32_t zz) {
   const uint32_t r = zz * a.ly + yy;
   return (uint64_t)fastdiv(r, a.lay.div_grows) * a.lay.gpad + (uint64_t)r * row_b;
+}
+
+// The same in 32 bits: within one population segment the offset is below
+// 4 GiB (a grouped buffer stays below 4 GiB; with one group r / grows = 0 and
+// one population array is below 4 GiB).
+RRLBM_INLINE __device__ uint32_t row_offset32(const FusedArgs& a, uint32_t row_b, uint32_t yy, uint32_t zz) {
+  const uint32_t r = zz * a.ly + yy;
+  return fastdiv(r, a.lay.div_grows) * (uint32_t)a.lay.gpad + r * row_b;
 }
 
 // Pull gather of the Q upwind populations of the cell (x, y, z), per-lane
@@ -533,15 +552,33 @@ RRLBM_INLINE __device__ void fused_body(const FusedArgs& a, uint32_t b) {
   C h[Q];
   [[maybe_unused]] _Float16 g[Q];
   if constexpr (ROWS) {
-    // nine scalar source-row bases, then the lane's x offset
+    // nine scalar source-row offsets, then population i at
+    // src + i*seg + row offset + the lane's x offset.  The population base
+    // advances by seg in a scalar register pair held opaque: left to the
+    // compiler, the 27 products i*seg of the gather and the 27 of the store are
+    // formed up front, overflow the scalar register file and spill into
+    // vector-register lanes (about 100 vector instructions per wavefront in
+    // the FP16 LES step; 1.5-3 % of the step time).
+    uint32_t ro[3][3];
+    static_for<3>([&](auto jy) {
+      static_for<3>([&](auto jz) {
+        constexpr int iy = decltype(jy)::value, iz = decltype(jz)::value;
+        ro[iy][iz] = row_offset32(a, row_b, ys[iy], zs[iz]);
+      });
+    });
+    uint64_t pb = reinterpret_cast<uint64_t>(src);
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-      const char* row = src + row_offset(a, row_b, ys[cy + 1], zs[cz + 1]);
+      const T v = *reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xb[cx + 1]);
       if constexpr (kRawF16)
-        g[i] = *reinterpret_cast<const _Float16*>(row + (uint64_t)i * a.lay.seg + xb[cx + 1]);
+        g[i] = v;
+      else if constexpr (S == kF16)
+        h[i] = C(v) * (C(L::w(i)) * inv_scale);
       else
-        h[i] = load_pop_b<S>(row, a.lay.seg, i, xb[cx + 1], C(L::w(i)) * inv_scale);
+        h[i] = C(v);
+      pb += a.lay.seg;
+      asm volatile("" : "+s"(pb));
     });
   } else {
     gather_lane<S, L>(a, x, y, z, h);
