@@ -463,23 +463,53 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27_o6(const Macro<C>& q, const NeqTensor<C>&
         TU = X.T2[jx] * Y.T1[jy] + X.T1[jx] * Y.T2[jy];
         if constexpr (fyz) Sxy = X.g[jx] + Y.g[jy];
       }
-      static_for<3>([&](auto jzc) {
-        constexpr int jz = decltype(jzc)::value;
-        constexpr int i = D3Q27::index(jx - 1, jy - 1, jz - 1);
-        const C E = Exy + Z.p[jz] * PP;
-        C val = dr_s + rho_s * E + Ap * Z.P[jz] + Bp * hval(jz) + Cp * Z.D[jz];
+      if constexpr (!fyz) {
+        // Force along x or none: with P_z = 1 + p_z the column over c_z is
+        //   val = al + be p_z + Cp D_z + ga T2_z + Bp h_z,
+        // whose z factors take at c_z = +-1 the values p = s +- t, D = 2u +- 1,
+        // T2 = s, h = 2 and at c_z = 0 p = T2 = -3/2 u^2, D = -u, h = -1
+        // (s = 3u^2, t = 3u): the column is an even part plus or minus an odd one.
+        C al = dr_s + rho_s * Exy + Ap;
+        C be = rho_s * PP + Ap;
+        C ga = C(0);
         if constexpr (fx) {
-          const C Qv = PP * Z.P[jz] - TT * Z.p[jz] - TU * Z.T2[jz];
-          if constexpr (fyz)
-            val += grho * (Sxy + Z.g[jz]) * Qv;
-          else
-            val += gxr * Qv;
+          al += gxr * PP;
+          be += gxr * (PP - TT);
+          ga = -gxr * TU;
         }
-        if constexpr (per_direction_w)
-          out[i] = val * C(D3Q27::w(i));
-        else
-          out[i] = val;
-      });
+        const C bg = be + ga;
+        const C uz = q.uz;
+        const C v0 = al + bg * Z.T2[1] - uz * Cp - Bp;
+        const C ev = al + bg * Z.T2[0] + (C(2) * uz) * Cp + C(2) * Bp;
+        const C od = be * Z.T1[2] + Cp;
+        const C col[3] = {ev - od, v0, ev + od};
+        static_for<3>([&](auto jzc) {
+          constexpr int jz = decltype(jzc)::value;
+          constexpr int i = D3Q27::index(jx - 1, jy - 1, jz - 1);
+          if constexpr (per_direction_w)
+            out[i] = col[jz] * C(D3Q27::w(i));
+          else
+            out[i] = col[jz];
+        });
+      } else {
+        static_for<3>([&](auto jzc) {
+          constexpr int jz = decltype(jzc)::value;
+          constexpr int i = D3Q27::index(jx - 1, jy - 1, jz - 1);
+          const C E = Exy + Z.p[jz] * PP;
+          C val = dr_s + rho_s * E + Ap * Z.P[jz] + Bp * hval(jz) + Cp * Z.D[jz];
+          if constexpr (fx) {
+            const C Qv = PP * Z.P[jz] - TT * Z.p[jz] - TU * Z.T2[jz];
+            if constexpr (fyz)
+              val += grho * (Sxy + Z.g[jz]) * Qv;
+            else
+              val += gxr * Qv;
+          }
+          if constexpr (per_direction_w)
+            out[i] = val * C(D3Q27::w(i));
+          else
+            out[i] = val;
+        });
+      }
     });
   });
 }
