@@ -627,6 +627,76 @@ def main_copy():
     print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n, "copies": 5}))
 
 
+def main_sizes(args):
+    """The fused step's cost as a function of grid size (library defaults, eager
+    launches, best of two contexts): FP16 RR+WALE, FP16 RR and FP32 RR+WALE on
+    channel grids of 0.98 M to 9.4 M cells, and a least-squares fit
+    us_per_step = t0 + c * cells (t0: the fixed cost of a launch on one stream;
+    c: the sustained cost per cell, whose bytes per lattice update / c is the
+    rate the kernel reaches on large grids)."""
+    import numpy as np
+
+    import rrlbm
+    from rrlbm import configs
+
+    rrlbm.load()
+    grids = [(192, 64, 80), (192, 64, 96), (192, 64, 128), (384, 64, 96), (384, 128, 96), (384, 128, 192)]
+    for st, les in ((2, True), (2, False), (1, True), (1, False)):
+        blu = 2 * 27 * STORAGE_BYTES[st] + (24 if les else 0)
+        pts = []
+        for lx, ly, lz in grids:
+            n = lx * ly * lz
+            best = 1e9
+            for _ in range(2):
+                s, _meta = configs.channel(storage=st, lx=lx, ly=ly, lz=lz, les=les)
+                s.fused_pack()
+                s.fused_step(args.warmup)
+                s.synchronize()
+                best = min(best, s.fused_step_timed(args.steps) / args.steps)
+                s.close()
+            us = best * 1e3
+            pts.append((n, us))
+            print(json.dumps({"storage": STORAGE_NAME[st], "les": les, "grid": [lx, ly, lz], "cells": n,
+                              "us_per_step": round(us, 2), "frac": round(n * blu / (us * 1e-6) / (PEAK_GBS * 1e9), 4)}),
+                  flush=True)
+        n_arr = np.array([p[0] for p in pts], dtype=float)
+        t_arr = np.array([p[1] for p in pts])
+        c, t0 = np.polyfit(n_arr, t_arr, 1)
+        print(json.dumps({"storage": STORAGE_NAME[st], "les": les, "fit_t0_us": round(float(t0), 2),
+                          "fit_ns_per_kcell": round(float(c) * 1e6, 3),
+                          "sustained_frac": round(blu / (c * 1e-6) / (PEAK_GBS * 1e9), 4)}), flush=True)
+
+
+def main_ceiling():
+    """What plain streams reach on this GPU, for comparison with the fused step:
+    torch's device-to-device copy (hipMemcpyAsync), a read-only reduction and a
+    write-only fill, each over 1 GiB buffers, median of 20 timed repetitions
+    (HIP events on torch's stream).  Bytes counted: read + written."""
+    import torch
+
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    out = {}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 8 * n), ("read_sum", lambda: torch.sum(a), 4 * n),
+                            ("write_fill", lambda: b.fill_(1.0), 4 * n),
+                            ("elementwise_add", lambda: torch.add(a, 1.0, out=b), 8 * n)):
+        fn()
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(20):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = sorted(ts)[len(ts) // 2]
+        out[name] = {"us": round(ms * 1e3, 1), "tb_per_s": round(moved / (ms * 1e-3) / 1e12, 3)}
+    print(json.dumps({"workload": "ceiling", "buffer_bytes": 4 * n, "results": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -638,7 +708,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "ensemble", "configs", "validate", "copy", "turbulence", "les-step"])
+                    choices=["channel", "ensemble", "configs", "validate", "copy", "ceiling", "sizes", "turbulence", "les-step"])
     ap.add_argument("--turb-steps", type=int, default=1000000)
     ap.add_argument("--turb-window", type=int, default=300000)
     ap.add_argument("--turb-gp-steps", type=int, default=50000)
@@ -661,6 +731,10 @@ def main():
         return main_ensemble(args)
     if args.workload == "copy":
         return main_copy()
+    if args.workload == "ceiling":
+        return main_ceiling()
+    if args.workload == "sizes":
+        return main_sizes(args)
     if args.workload == "configs":
         return main_configs(args)
     if args.workload == "turbulence":
