@@ -3,7 +3,8 @@ the grid size (the oracle is too slow at these sizes): the fused path against
 the drop-in kernels in the reference order at FP64, FP32 / FP16 against FP64
 on the same grid, conservation and bit-identity of the launch forms.
 
-Grids: the channel DNS 384 x 128 x 192 (9.4 M cells), the LES channel
+Grids: the channel DNS 384 x 128 x 192 (9.4 M cells; also under the BGK
+operator, the sixth-order RR terms and FDBC walls), the LES channel
 192 x 64 x 96, the 128^3 cavity, the 64-instance Taylor-Green ensemble
 (1.4 x 10^8 cells)."""
 import numpy as np
@@ -48,6 +49,42 @@ def test_cavity_fused_equals_reference_order(rr, npop):
     from rrlbm import configs
     ef, em = fused_vs_reference(rr, configs.cavity, 4, npop=npop)
     assert ef < 1e-13 and em < 1e-13, (ef, em)
+
+
+@pytest.mark.parametrize("op", ["bgk", "rr6", "fdbc"])
+def test_channel_operators_fused_equal_reference_order(rr, op):
+    """The channel grid under the BGK operator (BGK/main.cu:49-70 order), the RR
+    collision with the fourth- to sixth-order terms, and FDBC walls in place of
+    RRBC (main.cu:76-77): the fused FP64 step against the drop-in kernels."""
+    from rrlbm import configs
+    make, kw = {"bgk": (configs.channel_bgk, {}), "rr6": (configs.channel, {"rr_order": 6}),
+                "fdbc": (configs.channel, {"wall_rr": 2})}[op]
+    ef, em = fused_vs_reference(rr, make, 4, **kw)
+    print("%s channel 384x128x192, fused FP64 vs reference order: f %.2e, macroscopic %.2e" % (op, ef, em))
+    assert ef < 1e-13 and em < 1e-13, (ef, em)
+
+
+def test_channel_rr6_storage_precisions(rr):
+    """The sixth-order RR step in FP32 / FP16 storage against FP64 after 100
+    fused steps of the full channel."""
+    from rrlbm import configs
+    u = {}
+    for st in (0, 1, 2):
+        s, meta = configs.channel(storage=st, rr_order=6)
+        with s:
+            s.fused_pack()
+            s.fused_step(99)
+            d = s.fused_diag()
+            s.fused_unpack()
+            u[st] = (s.download(rr.Field.UX), d)
+    umax = np.abs(u[0][0]).max()
+    e32, e16 = maxabs(u[1][0], u[0][0]) / umax, maxabs(u[2][0], u[0][0]) / umax
+    b32, b16 = (abs(u[st][1]["u_sum"][0] / u[0][1]["u_sum"][0] - 1) for st in (1, 2))
+    print("channel RR6 FP32/FP16 vs FP64, 100 steps: u %.2e / %.2e, U_b %.2e / %.2e" % (e32, e16, b32, b16))
+    # measured u 4.7e-6 / 1.2e-3, U_b 2.5e-7 / 4.7e-8 of the FP64 values
+    assert e32 < 1.5e-5
+    assert e16 < 4e-3
+    assert b32 < 1e-6 and b16 < 4e-6
 
 
 def test_channel_dns_storage_precisions(rr):
