@@ -109,6 +109,29 @@ def test_channel_dns_storage_precisions(rr):
     assert b32 < 3e-6 and b16 < 4e-6
 
 
+def test_channel_les_storage_precisions(rr):
+    """The LES grid (192 x 64 x 96, WALE, lagged gradient) in FP32 / FP16
+    storage against FP64 after 200 fused steps."""
+    from rrlbm import configs
+    u = {}
+    for st in (0, 1, 2):
+        s, meta = configs.channel_les(storage=st)
+        with s:
+            s.fused_pack()
+            s.fused_step(199)
+            d = s.fused_diag()
+            s.fused_unpack()
+            u[st] = (s.download(rr.Field.UX), d)
+    umax = np.abs(u[0][0]).max()
+    e32, e16 = maxabs(u[1][0], u[0][0]) / umax, maxabs(u[2][0], u[0][0]) / umax
+    b32, b16 = (abs(u[st][1]["u_sum"][0] / u[0][1]["u_sum"][0] - 1) for st in (1, 2))
+    print("channel LES FP32/FP16 vs FP64, 200 steps: u %.2e / %.2e, U_b %.2e / %.2e" % (e32, e16, b32, b16))
+    # measured u 8.4e-6 / 1.4e-3, U_b 5.6e-7 / 5.5e-7 of the FP64 values
+    assert e32 < 2.5e-5
+    assert e16 < 4.5e-3
+    assert b32 < 2e-6 and b16 < 2e-6
+
+
 def test_ensemble_conservation_and_launch_forms(rr):
     """The 64-instance ensemble: batched = per-instance launches bit for bit
     (FP32), mass and momentum of every periodic box conserved, kinetic energy
