@@ -42,15 +42,19 @@ STORAGE_BYTES = {0: 8, 1: 4, 2: 2}
 STORAGE_NAME = {0: "f64", 1: "f32", 2: "f16"}
 
 
-def run_variant(rrlbm, storage, steps, warmup, tune=True, les=False):
+def run_variant(rrlbm, storage, steps, warmup, tune=True, les=False, two_step=False):
     """One storage variant of the channel step on the full grid (les: RR+WALE,
-    the one-launch lagged-gradient step); MLUPS over the wall clock, the
-    fused kernel's mean duration from HIP events on the library's stream."""
+    the one-launch lagged-gradient step; two_step: two steps per launch with z
+    chunks of 16 planes instead of the library default, without tuning); MLUPS
+    over the wall clock, the fused kernel's mean duration per step from HIP
+    events on the library's stream."""
+    extra = dict(steps_per_pass=2, pass_planes=16) if two_step else {}
     s = rrlbm.Solver(lx=LX, ly=LY, lz=LZ, npop=27, storage=storage, tau=TAU, force=(FORCE, 0.0, 0.0),
-                     wall_y=1, wall_rr=1, use_les=int(les), cw=0.5)
+                     wall_y=1, wall_rr=1, use_les=int(les), cw=0.5, **extra)
     try:
         s.init_channel(U_TAU, VISC, SEED, 0.1, 0.01)
         tuning = s.autotune(trial_steps=10) if tune else None
+        spp = s.params.steps_per_pass
         s.fused_pack()
         s.fused_step(warmup)
         s.synchronize()
@@ -63,14 +67,16 @@ def run_variant(rrlbm, storage, steps, warmup, tune=True, les=False):
         s.close()
     n = LX * LY * LZ
     mlups = n * steps / wall / 1e6
-    per_launch = kernel_ms / 1e3 / steps
+    per_step = kernel_ms / 1e3 / steps
     # LES: + the velocity field (3 components in the compute type) read once and written once
     blu = 2 * 27 * STORAGE_BYTES[storage] + (6 * (8 if storage == 0 else 4) if les else 0)
-    achieved = blu * n / per_launch / 1e9
+    # a launch of `spp` steps moves spp * blu * n algorithmic bytes in spp * per_step
+    achieved = blu * n / per_step / 1e9
     return {
         "mlups": mlups,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms": kernel_ms / steps,
+        "steps_per_launch": spp,
         "bytes_per_lu": blu,
         "achieved_gbs": achieved,
         "frac": achieved / PEAK_GBS,
@@ -80,14 +86,15 @@ def run_variant(rrlbm, storage, steps, warmup, tune=True, les=False):
     }
 
 
-def traffic_from_profiles(storage):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+def traffic_from_profiles(storage, two_step=False):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any
+    (two_step: of the two-steps-per-launch kernel, whose launch covers two steps)."""
     path = os.path.join(REPO, "profiles", "fused_channel_384x128x192_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
-    return t.get(STORAGE_NAME[storage], {}).get("hbm_bytes_per_launch")
+    return t.get(STORAGE_NAME[storage] + ("_two_step" if two_step else ""), {}).get("hbm_bytes_per_launch")
 
 
 def cpu_model():
@@ -232,9 +239,12 @@ def timed(s, steps, warmup, tune=True, graph=False):
     return s.n * steps / wall / 1e6, ms / steps, tuning
 
 
+TUNING_KEYS = ("pop_group_rows", "pop_pad_bytes", "cells_per_thread", "xcd_remap", "persistent_wg_per_cu",
+               "steps_per_pass", "pass_planes")
+
+
 def shape_of(t):
-    return None if t is None else {k: t[k] for k in ("pop_group_rows", "pop_pad_bytes", "cells_per_thread", "xcd_remap",
-                                                     "persistent_wg_per_cu")}
+    return None if t is None else {k: t[k] for k in TUNING_KEYS}
 
 
 def line_for(metric, workload, mlups, ms, blu, dtype, steps, warmup, extra):
@@ -705,6 +715,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-les", action="store_true", help="channel workload: leave out the RR+WALE variants")
     ap.add_argument("--no-tune", action="store_true", help="library default layout and launch shape (no autotuning)")
+    ap.add_argument("--two-step", action="store_true",
+                    help="channel workload: FP32/FP64 variants with two steps per launch (z chunks of 16 planes), "
+                         "no autotuning of those variants")
     ap.add_argument("--no-graph", action="store_true", help="configs workload: eager launches only (no HIP graphs)")
     ap.add_argument("--variants", default="1,2,0", help="storage variants; the first is the headline")
     ap.add_argument("--workload", default="channel",
@@ -747,7 +760,8 @@ def main():
 
     rrlbm.load()
     variants = [int(v) for v in args.variants.split(",")]
-    res = {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune) for st in variants}
+    res = {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune,
+                           two_step=args.two_step and st != 2) for st in variants}
     les = {} if args.no_les else {st: run_variant(rrlbm, st, args.steps, args.warmup, tune=not args.no_tune, les=True)
                                   for st in (1, 2)}
     head = res[variants[0]]
@@ -770,16 +784,17 @@ def main():
                    "lx": LX, "ly": LY, "lz": LZ, "npop": 27, "storage": STORAGE_NAME[st0], "u_tau": U_TAU,
                    "tau": TAU, "force_x": FORCE, "les": 0},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["frac"], 4), "traffic": traffic_from_profiles(st0),
-                     "bytes_per_lu": head["bytes_per_lu"], "kernel_ms": round(head["kernel_ms"], 5)},
+                     "frac": round(head["frac"], 4),
+                     "traffic": traffic_from_profiles(st0, head["steps_per_launch"] == 2),
+                     "bytes_per_lu": head["bytes_per_lu"], "kernel_ms": round(head["kernel_ms"], 5),
+                     "steps_per_launch": head["steps_per_launch"],
+                     "launch_ms": round(head["kernel_ms"] * head["steps_per_launch"], 5)},
         "variants": {STORAGE_NAME[st]: {"mlups": round(r["mlups"], 1), "frac": round(r["frac"], 4),
                                          "achieved_gbs": round(r["achieved_gbs"], 1),
                                          "bytes_per_lu": r["bytes_per_lu"], "kernel_ms": round(r["kernel_ms"], 5),
                                          "ubulk_over_utau": round(r["ubulk_over_utau"], 4),
-                                         "tuning": None if r["tuning"] is None else {
-                                             k: r["tuning"][k] for k in ("pop_group_rows", "pop_pad_bytes",
-                                                                         "cells_per_thread", "xcd_remap",
-                                                                         "persistent_wg_per_cu")}}
+                                         "steps_per_launch": r["steps_per_launch"],
+                                         "tuning": shape_of(r["tuning"])}
                      for st, r in res.items()},
         # the same grid with WALE LES (lagged gradient, one launch per step)
         "variants_les": {STORAGE_NAME[st]: {"mlups": round(r["mlups"], 1), "frac": round(r["frac"], 4),

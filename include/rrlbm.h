@@ -128,6 +128,13 @@ typedef struct rrlbm_params {
                                    runs it, or 6 with the fourth- to sixth-order Hermite terms the
                                    reference keeps commented out (collision.cu:48-49,62-67,95-104,
                                    112-116; BC.cu:37-38,50-54,79-88,97-101); 6: D3Q27, RR, no LES */
+    int32_t steps_per_pass;     /* fused steps per launch: 0 or 1 = one; 2 = two steps in one pass
+                                   over the populations (the first step's state stays on chip), for
+                                   the D3Q27 third-order RR step without LES in FP32 or FP64
+                                   storage with lx a multiple of 64 and ly of 4; an odd step count
+                                   and forcing_GP steps take single launches */
+    int32_t pass_planes;        /* steps_per_pass 2: z planes per workgroup's chunk (0 = library
+                                   default: at least six chunks per compute unit) */
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;
@@ -277,11 +284,14 @@ typedef struct rrlbm_tuning {
     int32_t candidates;           /* configurations timed; 0 = taken from this process's cache */
     double ms_per_step;           /* fused step time of the choice in the confirmation trial */
     double ms_per_step_default;   /* the library default's time in the same trial */
+    int32_t steps_per_pass;       /* as in rrlbm_params (value used) */
+    int32_t pass_planes;
 } rrlbm_tuning;
 /* Times ctx's fused step over candidate population layouts (rows per group,
  * padding) and launch shapes (1 or 2 cells per thread, block order over the
  * XCDs: linear, slabs, runs of 4, 16, 32 or 64 blocks; one workgroup per block or a
- * persistent grid of 4 or 8 per compute unit), best of three interleaved
+ * persistent grid of 4 or 8 per compute unit; where the configuration allows, two
+ * steps per launch with z chunks of 8, 16 or 24 planes), best of three interleaved
  * rounds of trial_steps steps each; then times the fastest against the
  * library default in rounds of 3 * trial_steps and keeps the default unless
  * the fastest still beats it.  The choice applies to ctx and is cached per

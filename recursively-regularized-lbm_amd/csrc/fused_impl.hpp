@@ -193,29 +193,14 @@ RRLBM_INLINE __device__ void regularise_face(C (&h)[L::Q], C Fx, C Fy, C Fz) {
   });
 }
 
-// Bounce-back value of population i entering through `face` at this cell:
-// post-collision opposite population of the same cell (+ moving-lid term
-// 6 w_i c_i.u_lid, Cavity/streaming.f90:142-152).
-template <int S, class L>
-RRLBM_INLINE __device__ typename StoreT<S>::C bb_value(const FusedArgs& a, const void* base, int i, int face,
-                                                       uint32_t off, typename StoreT<S>::C inv_scale) {
-  using C = typename StoreT<S>::C;
-  const int o = L::opp(i);
-  C v = load_pop_b<S>(base, a.lay.seg, o, off, C(L::w(i)) * inv_scale);
-  if (face == a.lid_face)
-    v += C(6.0 * L::w(i)) * (C(L::cx(i)) * C(a.lid_x) + C(L::cy(i)) * C(a.lid_y) + C(L::cz(i)) * C(a.lid_z));
-  return v;
-}
-
-// off: this cell's byte offset from `base` in the source buffer (without i*seg);
-// O6: regularise with the fourth- to sixth-order terms
-template <int S, class L, int FM, bool O6 = false>
-RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
-                                            uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off,
-                                            typename StoreT<S>::C Fx, typename StoreT<S>::C Fy,
-                                            typename StoreT<S>::C Fz) {
-  using C = typename StoreT<S>::C;
-  const C inv_scale = inv_scale_of<S>(a);
+// Wall treatment of one cell (h: its pulled populations, replaced in place).
+// own(o): the cell's own post-collision population o of the previous step
+// (unscaled, shifted), the source of the half-way bounce-back of population
+// opp(o) (+ moving-lid term 6 w_i c_i.u_lid, Cavity/streaming.f90:142-152).
+// O6: regularise with the fourth- to sixth-order terms.
+template <class L, typename C, int FM, bool O6, class Own>
+RRLBM_INLINE __device__ void wall_treatment_own(const FusedArgs& a, C (&h)[L::Q], uint32_t x, uint32_t y,
+                                                uint32_t z, Own&& own, C Fx, C Fy, C Fz) {
   const bool on[6] = {(a.walls & 1) && x == 0,        (a.walls & 1) && x == a.lx - 1,
                       (a.walls & 2) && y == 0,        (a.walls & 2) && y == a.ly - 1,
                       (a.walls & 4) && z == 0,        (a.walls & 4) && z == a.lz - 1};
@@ -230,10 +215,29 @@ RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<
     if (!on[face]) return;
     static_for<L::Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) h[i] = bb_value<S, L>(a, base, i, face, off, inv_scale);
+      if constexpr (enters(face, L::cx(i), L::cy(i), L::cz(i))) {
+        C v = own(L::opp(i));
+        if (face == a.lid_face)
+          v += C(6.0 * L::w(i)) * (C(L::cx(i)) * C(a.lid_x) + C(L::cy(i)) * C(a.lid_y) + C(L::cz(i)) * C(a.lid_z));
+        h[i] = v;
+      }
     });
     if (a.wall_rr) regularise_face<L, C, face, FM, O6>(h, Fx, Fy, Fz);
   });
+}
+
+// The same with the previous step's populations in the source buffer:
+// `base` + population offset + off (this cell's byte offset, without i*seg).
+template <int S, class L, int FM, bool O6 = false>
+RRLBM_INLINE __device__ void wall_treatment(const FusedArgs& a, typename StoreT<S>::C (&h)[L::Q],
+                                            uint32_t x, uint32_t y, uint32_t z, const void* base, uint32_t off,
+                                            typename StoreT<S>::C Fx, typename StoreT<S>::C Fy,
+                                            typename StoreT<S>::C Fz) {
+  using C = typename StoreT<S>::C;
+  const C inv_scale = inv_scale_of<S>(a);
+  wall_treatment_own<L, C, FM, O6>(
+      a, h, x, y, z, [&](int o) { return load_pop_b<S>(base, a.lay.seg, o, off, C(L::w(o)) * inv_scale); }, Fx, Fy,
+      Fz);
 }
 
 // LES velocity of one cell: the three components are stored together (AoS,
@@ -846,6 +850,225 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
   } else {
     const uint32_t b = block_of(a, blockIdx.x);
     if (b < a.nblocks) fused_pair_body<S, WALLS, FM>(a, b);
+  }
+}
+
+// ---- Two steps per launch ----------------------------------------------
+//
+// The stored state of step t (post-collision populations in a.src) goes to
+// that of step t + 2 (a.dst) in one pass, so each population crosses HBM once
+// per two steps.  A workgroup owns a tile of kT2X x kT2Y cells in (x, y) and a
+// chunk of planes [z0, z1) in z, and marches k = z0 - 1 .. z1:
+//   A. step t+1 at plane k on the tile and a one-cell halo around it
+//      ((kT2X + 2) x (kT2Y + 2) cells), gathered from a.src as the one-step
+//      kernel does, into registers; the populations with c_z = -1 go to LDS;
+//   B. step t+2 at plane k - 1 on the tile, gathered from the LDS ring
+//      (c_z = +1 from plane k - 2, 0 from k - 1, -1 from k), stored to a.dst;
+//   C. the populations of plane k with c_z = 0 and +1 go to LDS.
+// The ring holds plane k - 1 in full while B runs (the wall cells' bounce-back
+// reads their own populations of that plane): c_z = -1 and +1 in two slots
+// each, c_z = 0 in one, 45 x 396 values.  Both steps call the one-step
+// kernel's per-cell functions, so the result equals two one-step launches bit
+// for bit.  Planes k - 1 and z1 of step t+1 are computed by both chunks that
+// border them.
+constexpr uint32_t kH2X = kT2X + 2, kH2Y = kT2Y + 2, kH2N = kH2X * kH2Y;
+
+// Index of population i among the nine with the same c_z.
+template <class L>
+RRLBM_HD constexpr int zgroup_index(int i) {
+  return (L::cx(i) + 1) * 3 + (L::cy(i) + 1);
+}
+
+// Occupancy: FP64 (142 KB of LDS) runs one workgroup per compute unit.  FP32
+// (71 KB) prefetches the next plane's step-1 gather into registers while step
+// 2 runs, which needs more registers than two workgroups of the walled kernel
+// allow (one workgroup, 156 VGPRs; measured faster than two without the
+// prefetch); the periodic kernel keeps two (4 wavefronts per SIMD).
+template <int S, bool WALLS, int FM>
+__global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu((S == kF32 && !WALLS) ? 4 : 2))) void
+fused2_kernel(FusedArgs a, Fused2Args t) {
+  using C = typename StoreT<S>::C;
+  using T = typename StoreT<S>::T;
+  using L = D3Q27;
+  constexpr int Q = 27;
+  static_assert(S != kF16, "two steps per launch: FP32 or FP64 storage");
+  // work item: XCD (blockIdx.x mod 8) takes a contiguous run of items, x
+  // fastest, so the workgroups an XCD runs at one time cover whole rows of
+  // tiles and share their halo cache lines in its L2
+  const uint32_t f = (blockIdx.x & 7u) * t.per_xcd + (blockIdx.x >> 3);
+  if (f >= t.items) return;  // the whole workgroup: no barrier is left half-reached
+  const uint32_t tx = f % t.ntx, ty = (f / t.ntx) % t.nty, ch = f / (t.nty * t.ntx);
+  const uint32_t x0 = tx * kT2X, y0 = ty * kT2Y, z0 = ch * t.zc;
+  const uint32_t zlen = min(t.zc, a.lz - z0);
+
+  __shared__ C sM[2][9][kH2N], sZ[9][kH2N], sP[2][9][kH2N];
+
+  // w is uniform across a wavefront; readfirstlane lets the compiler keep the
+  // store row's base in scalar registers
+  const uint32_t tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
+  // step t+1: wavefronts 0-5 take halo row w, columns 1..64; wavefront 6
+  // lanes 0-11 the two halo columns 0 and 65 of the six rows
+  uint32_t hx, hy;
+  bool h_on = true;
+  if (w < kH2Y) {
+    hy = w;
+    hx = lane + 1;
+  } else {
+    h_on = lane < 2 * kH2Y;
+    hy = h_on ? lane >> 1 : 0u;
+    hx = (lane & 1u) ? kH2X - 1 : 0u;
+  }
+  const uint32_t hidx = hy * kH2X + hx;
+  const uint32_t gx = (x0 + hx + a.lx - 1) % a.lx, gy = (y0 + hy + a.ly - 1) % a.ly;
+  // step t+2: wavefronts 0-3, row w of the tile, column lane
+  const bool i_on = w < kT2Y;
+  const uint32_t iidx = (w + 1) * kH2X + lane + 1;
+  const uint32_t ix = x0 + lane, iy = y0 + w;
+
+  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  const C omega = C(a.omega);
+  constexpr uint32_t sz = sizeof(T);
+  const uint32_t row_b = a.lx * sz;
+  auto on_wall = [&](uint32_t x, uint32_t y, uint32_t z) {
+    return ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
+           ((a.walls & 4) && (z == 0 || z == a.lz - 1));
+  };
+  // moments, macrovar and the RR collision of a cell whose pulled (and wall-
+  // treated) populations are h
+  auto collide = [&](C (&h)[Q], C (&out)[Q]) {
+    const Moments<C> m = moments_q27<C>(h);
+    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
+    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+  };
+
+  // Halo rows: (y, z) is uniform across the wavefront, so the nine source rows
+  // of the step-1 gather are scalar bases and each lane adds its x offset.
+  // FP32 issues the gather of the next plane before step 2 of this one.
+  constexpr bool kPrefetch = S == kF32;
+  const uint32_t gyr = (y0 + w + a.ly - 1) % a.ly;  // = gy in the halo-row wavefronts, uniform
+  const uint32_t ysr[3] = {gyr + 1 == a.ly ? 0u : gyr + 1, gyr, gyr == 0 ? a.ly - 1 : gyr - 1};
+  const uint32_t xbr[3] = {(gx + 1 == a.lx ? 0u : gx + 1) * sz, gx * sz, (gx == 0 ? a.lx - 1 : gx - 1) * sz};
+  auto row_gather = [&](uint32_t kk, C (&dst)[Q]) {
+    const uint32_t zs[3] = {kk + 1 == a.lz ? 0u : kk + 1, kk, kk == 0 ? a.lz - 1 : kk - 1};
+    uint32_t ro[3][3];
+    static_for<3>([&](auto jy) {
+      static_for<3>([&](auto jz) {
+        constexpr int iy2 = decltype(jy)::value, iz2 = decltype(jz)::value;
+        ro[iy2][iz2] = row_offset32(a, row_b, ysr[iy2], zs[iz2]);
+      });
+    });
+    uint64_t pb = reinterpret_cast<uint64_t>(a.src);
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xbr[cx + 1]));
+      pb += a.lay.seg;
+      asm volatile("" : "+s"(pb));
+    });
+  };
+  [[maybe_unused]] C hn[Q];
+  if constexpr (kPrefetch) {
+    if (w < kH2Y) row_gather((z0 + a.lz - 1) % a.lz, hn);
+  }
+
+  for (uint32_t it = 0; it < zlen + 2; ++it) {
+    const uint32_t k = (z0 + a.lz - 1 + it) % a.lz;  // plane of step t+1
+    const uint32_t p = it & 1u;                      // ring slot of plane k (k - 2 before C)
+    // A. step t+1 at plane k
+    C o1[Q];
+    if (h_on) {
+      C h[Q];
+      if (w < kH2Y) {
+        if constexpr (kPrefetch)
+          static_for<Q>([&](auto ic) { h[decltype(ic)::value] = hn[decltype(ic)::value]; });
+        else
+          row_gather(k, h);
+      } else {
+        gather_lane<S, L>(a, gx, gy, k, h);
+      }
+      if constexpr (WALLS) {
+        if (on_wall(gx, gy, k)) {
+          const uint32_t off = cell_off<S>(a.lay, (k * a.ly + gy) * a.lx + gx);
+          wall_treatment<S, L, FM>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
+        }
+      }
+      collide(h, o1);
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (L::cz(i) < 0) sM[p][zgroup_index<L>(i)][hidx] = o1[i];
+      });
+    }
+    if constexpr (kPrefetch) {
+      if (w < kH2Y && it + 1 < zlen + 2) row_gather(k + 1 == a.lz ? 0u : k + 1, hn);
+    }
+    __syncthreads();
+    // B. step t+2 at plane k - 1
+    if (it >= 2 && i_on) {
+      const uint32_t q = 1u - p;  // ring slot of plane k - 1
+      const uint32_t kz = (k + a.lz - 1) % a.lz;
+      C h[Q];
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), j = zgroup_index<L>(i);
+        const uint32_t src = iidx - (uint32_t)(cy * (int)kH2X + cx);
+        if constexpr (cz > 0)
+          h[i] = sP[p][j][src];
+        else if constexpr (cz == 0)
+          h[i] = sZ[j][src];
+        else
+          h[i] = sM[p][j][src];
+      });
+      if constexpr (WALLS) {
+        if (on_wall(ix, iy, kz)) {
+          wall_treatment_own<L, C, FM, false>(
+              a, h, ix, iy, kz,
+              [&](int o) -> C {
+                const int j = (L::cx(o) + 1) * 3 + (L::cy(o) + 1);
+                return L::cz(o) > 0 ? sP[q][j][iidx] : (L::cz(o) == 0 ? sZ[j][iidx] : sM[q][j][iidx]);
+              },
+              Fx, Fy, Fz);
+        }
+      }
+      C o2[Q];
+      collide(h, o2);
+      char* own_dst = reinterpret_cast<char*>(a.dst) + row_offset(a, row_b, iy, kz);
+      store_cell<S, Q>(a, own_dst, ix * sz, (kz * a.ly + iy) * a.lx + ix, o2);
+    }
+    __syncthreads();
+    // C. the rest of plane k into the ring
+    if (h_on) {
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (L::cz(i) == 0)
+          sZ[zgroup_index<L>(i)][hidx] = o1[i];
+        else if constexpr (L::cz(i) > 0)
+          sP[p][zgroup_index<L>(i)][hidx] = o1[i];
+      });
+    }
+  }
+}
+
+template <int S>
+hipError_t launch_fused2_storage(bool walls, int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  if constexpr (S == kF16) {
+    return hipErrorInvalidValue;
+  } else {
+    const dim3 grid(8u * t.per_xcd), block(kT2Threads);
+    if (walls) {
+      switch (fm) {
+        case kForceNone: hipLaunchKernelGGL((fused2_kernel<S, true, kForceNone>), grid, block, 0, s, a, t); break;
+        case kForceX: hipLaunchKernelGGL((fused2_kernel<S, true, kForceX>), grid, block, 0, s, a, t); break;
+        default: hipLaunchKernelGGL((fused2_kernel<S, true, kForceAll>), grid, block, 0, s, a, t); break;
+      }
+    } else {
+      switch (fm) {
+        case kForceNone: hipLaunchKernelGGL((fused2_kernel<S, false, kForceNone>), grid, block, 0, s, a, t); break;
+        case kForceX: hipLaunchKernelGGL((fused2_kernel<S, false, kForceX>), grid, block, 0, s, a, t); break;
+        default: hipLaunchKernelGGL((fused2_kernel<S, false, kForceAll>), grid, block, 0, s, a, t); break;
+      }
+    }
+    return hipGetLastError();
   }
 }
 

@@ -39,6 +39,10 @@ hipError_t launch_batched_s(Tag, const FusedArgs* table, const uint2* work, uint
   return launch_batched_s<kS>(table, work, ntiles, grid, parity, step, step_base, s);
 }
 
+hipError_t launch_fused2_s(Tag, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  return launch_fused2_storage<kS>(walls, force_mode_of(a), a, t, s);
+}
+
 hipError_t launch_fdbc_rows_s(Tag, const FusedArgs& a, hipStream_t s) {
   const dim3 grid((4 * a.lx * a.lz + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(fdbc_rows_kernel<kS>, grid, dim3(kBlock), 0, s, a);

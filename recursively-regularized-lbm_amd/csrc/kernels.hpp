@@ -140,6 +140,20 @@ struct FusedArgs {
   PopLayout lay;        // population buffer layout
 };
 
+// Two steps per launch (fused2_kernel): the domain is cut into tiles of
+// kT2X x kT2Y cells in (x, y) and chunks of `zc` planes in z; a workgroup
+// marches through its tile's chunk plane by plane, computing the first step
+// on the tile and a one-cell halo around it into a ring of planes in LDS and
+// the second step on the tile from that ring.
+constexpr uint32_t kT2X = 64, kT2Y = 4;
+constexpr uint32_t kT2Threads = 448;  // 7 wavefronts: 6 halo rows + the two halo columns
+struct Fused2Args {
+  uint32_t ntx, nty;  // tiles along x (lx / kT2X) and y (ly / kT2Y)
+  uint32_t zc, nzc;   // planes per chunk, chunks (the last may be shorter)
+  uint32_t items;     // ntx * nty * nzc
+  uint32_t per_xcd;   // items per XCD: XCD j takes items [j * per_xcd, (j + 1) * per_xcd)
+};
+
 // Launchers of one storage type's fused kernels (fused_storage.hip, built once
 // per storage type); the storage is a tag type, one overload per translation unit.
 template <int S>
@@ -153,7 +167,8 @@ struct StorageTag {};
   hipError_t launch_batched_s(StorageTag<S>, const FusedArgs* table, const uint2* work, uint32_t ntiles,            \
                               uint32_t grid, uint32_t parity, uint32_t step, const uint32_t* step_base,             \
                               hipStream_t s);                                                                       \
-  hipError_t launch_fdbc_rows_s(StorageTag<S>, const FusedArgs& a, hipStream_t s);
+  hipError_t launch_fdbc_rows_s(StorageTag<S>, const FusedArgs& a, hipStream_t s);                                 \
+  hipError_t launch_fused2_s(StorageTag<S>, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s);
 RRLBM_STORAGE_LAUNCHERS(kF64)
 RRLBM_STORAGE_LAUNCHERS(kF32)
 RRLBM_STORAGE_LAUNCHERS(kF16)
@@ -169,6 +184,9 @@ hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const Fuse
                            hipStream_t s);
 // FDBC pre-pass: macroscopic velocity of rows 1, 2, LY-2, LY-3 into a.u_wall.
 hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s);
+// Two steps in one launch (D3Q27 RR without LES or per-cell force, FP32 / FP64
+// storage, lx a multiple of kT2X, ly of kT2Y): a.src -> a.dst.
+hipError_t launch_fused2(int storage, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list;

@@ -35,6 +35,14 @@ hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32
   }
 }
 
+hipError_t launch_fused2(int storage, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  switch (storage) {
+    case kF64: return launch_fused2_s(StorageTag<kF64>{}, walls, a, t, s);
+    case kF32: return launch_fused2_s(StorageTag<kF32>{}, walls, a, t, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_batched(int storage, const FusedArgs* table, const uint2* work, uint32_t ntiles, uint32_t grid,
                           uint32_t parity, uint32_t step, const uint32_t* step_base, hipStream_t s) {
   switch (storage) {

@@ -61,6 +61,7 @@ struct rrlbm_ctx {
   int graph_steps = 0;
   bool graph_dev_step = false;  // the captured steps read d_step (advanced by every replay)
   void* graph_pop_a = nullptr;  // buffer the captured sequence starts from
+  Fused2Args pass{};   // steps_per_pass 2: tiles and z chunks of the two-step launch
   bool ref_ok = true;  // reference-layout fields allocated
   size_t bytes = 0;
   std::string err;
@@ -224,6 +225,33 @@ FusedArgs fused_args(rrlbm_ctx* c) {
   return a;
 }
 
+// Configurations the two-step kernel covers.
+bool two_step_ok(const rrlbm_params& p) {
+  return p.npop == 27 && !p.use_les && p.collision_model == RRLBM_COLLISION_RR && p.rr_order != 6 &&
+         p.wall_rr != 2 && p.storage != RRLBM_STORE_F16 && p.lx % (int)kT2X == 0 && p.ly % (int)kT2Y == 0;
+}
+
+// Geometry of the two-step launch: tiles of kT2X x kT2Y cells, z chunks of
+// pass_planes planes (default: the fewest chunks that give at least six work
+// items per compute unit), the items dealt to the 8 XCDs in contiguous runs.
+Fused2Args pass_geometry(const rrlbm_params& p, int cus) {
+  Fused2Args t{};
+  t.ntx = (uint32_t)p.lx / kT2X;
+  t.nty = (uint32_t)p.ly / kT2Y;
+  const uint32_t tiles = t.ntx * t.nty, lz = (uint32_t)p.lz;
+  uint32_t zc = (uint32_t)p.pass_planes;
+  if (zc == 0) {
+    const uint32_t want = (6u * (uint32_t)cus + tiles - 1) / tiles;
+    const uint32_t nzc = std::max(1u, std::min(lz, want));
+    zc = (lz + nzc - 1) / nzc;
+  }
+  t.zc = std::min(zc, lz);
+  t.nzc = (lz + t.zc - 1) / t.zc;
+  t.items = tiles * t.nzc;
+  t.per_xcd = (t.items + 7) / 8;
+  return t;
+}
+
 // Workgroups of a step launch over a.tiles blocks: one per block, or a
 // persistent grid of persistent_wg_per_cu workgroups per compute unit striding
 // over them.  Only the step kernels stride; the macro, finish and diagnostics
@@ -274,7 +302,18 @@ int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
 
 // next: the force of the collided steps (nullptr: the uniform force)
 int fused_steps(rrlbm_ctx* c, int nsteps, const rrlbm_ctx::Forcing* next = nullptr) {
-  for (int s = 0; s < nsteps; ++s) {
+  int s = 0;
+  if (c->p.steps_per_pass == 2 && !next && !c->force_cur.gp) {
+    // pairs of steps in one launch each (uniform force throughout)
+    for (; s + 2 <= nsteps; s += 2) {
+      const FusedArgs a = fused_args(c);
+      hipError_t e = launch_fused2(c->p.storage, has_walls(c->p), a, c->pass, c->stream);
+      if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "two-step kernel launch failed: %s", hipGetErrorString(e));
+      c->sr_step += 2;
+      std::swap(c->pop_a, c->pop_b);
+    }
+  }
+  for (; s < nsteps; ++s) {
     FusedArgs a = fused_args(c);
     if (next && next->gp) {
       a.gp_on[1] = 1;
@@ -486,6 +525,13 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
     return fail(nullptr, RRLBM_E_INVALID, "cells_per_thread must be 0, 1 or 2");
   if (p.persistent_wg_per_cu < 0 || p.persistent_wg_per_cu > 64)
     return fail(nullptr, RRLBM_E_INVALID, "persistent_wg_per_cu must be in [0, 64]");
+  if (p.steps_per_pass < 0 || p.steps_per_pass > 2) return fail(nullptr, RRLBM_E_INVALID, "steps_per_pass must be 0, 1 or 2");
+  if (p.steps_per_pass == 0) p.steps_per_pass = 1;
+  if (p.steps_per_pass == 2 && !two_step_ok(p))
+    return fail(nullptr, RRLBM_E_UNSUPPORTED,
+                "two steps per pass: the D3Q27 third-order RR step without LES or FDBC, FP32 or FP64 storage, "
+                "lx a multiple of %u and ly of %u", kT2X, kT2Y);
+  if (p.pass_planes < 0) return fail(nullptr, RRLBM_E_INVALID, "pass_planes must be >= 0");
   if (p.f16_rounding != RRLBM_F16_STOCHASTIC && p.f16_rounding != RRLBM_F16_NEAREST)
     return fail(nullptr, RRLBM_E_INVALID, "f16_rounding must be 0 (stochastic) or 1 (nearest)");
   if (!(p.f16_scale > 0.0)) return fail(nullptr, RRLBM_E_INVALID, "f16_scale must be positive");
@@ -522,6 +568,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) return bail(fail(c, RRLBM_E_HIP, "HIP setup failed: %s", hipGetErrorString(e)));
+  if (p.steps_per_pass == 2) c->pass = pass_geometry(p, c->cus);
 
   const size_t n = c->n, qn = (size_t)c->q * n;
   int rc = RRLBM_OK;
@@ -1095,9 +1142,9 @@ int force_mode_key(const rrlbm_params& p) {
 
 std::string tune_key(const rrlbm_params& p) {
   char buf[256];
-  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d m%d o%d", p.lx, p.ly, p.lz, p.npop,
-           p.storage, p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged,
-           force_mode_key(p), p.collision_model, p.rr_order);
+  snprintf(buf, sizeof buf, "%dx%dx%d q%d s%d w%d%d%d rr%d lid%d les%d/%d f%d m%d o%d p%d/%d", p.lx, p.ly, p.lz,
+           p.npop, p.storage, p.wall_x, p.wall_y, p.wall_z, p.wall_rr, p.lid_face >= 0, p.use_les, p.les_lagged,
+           force_mode_key(p), p.collision_model, p.rr_order, p.steps_per_pass, p.pass_planes);
   return buf;
 }
 
@@ -1137,6 +1184,7 @@ int apply_layout(rrlbm_ctx* c, int group_rows, int pad) {
 
 struct Candidate {
   int group_rows, pad, cpt, remap, pers;
+  int spp = 1, planes = 0;  // steps per pass, z planes per chunk of the two-step launch
 };
 
 int apply_candidate(rrlbm_ctx* c, const Candidate& k) {
@@ -1145,6 +1193,10 @@ int apply_candidate(rrlbm_ctx* c, const Candidate& k) {
   c->p.cells_per_thread = k.cpt;
   c->p.xcd_remap = k.remap;
   c->p.persistent_wg_per_cu = k.pers;
+  if (c->p.steps_per_pass != k.spp || c->p.pass_planes != k.planes) drop_graph(c);
+  c->p.steps_per_pass = k.spp;
+  c->p.pass_planes = k.planes;
+  if (k.spp == 2) c->pass = pass_geometry(c->p, c->cus);
   return RRLBM_OK;
 }
 
@@ -1204,7 +1256,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
       *out = it->second;
       out->candidates = 0;
       return apply_candidate(c, Candidate{out->pop_group_rows, out->pop_pad_bytes, out->cells_per_thread,
-                                          out->xcd_remap, out->persistent_wg_per_cu});
+                                          out->xcd_remap, out->persistent_wg_per_cu, out->steps_per_pass,
+                                          out->pass_planes});
     }
   }
   const uint32_t sr0 = c->sr_step;
@@ -1238,6 +1291,10 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
         if (cpt == 2 && !pair_ok) continue;
         shapes.push_back({base.group_rows, base.pad, cpt, remap, pers});
       }
+  // two steps per launch, z chunks of 8, 16 or 24 planes
+  if (two_step_ok(c->p))
+    for (int planes : {8, 16, 24})
+      if (planes <= c->p.lz) shapes.push_back({base.group_rows, base.pad, 1, base.remap, 0, 2, planes});
   std::vector<double> ms2;
   int b2 = 0;
   rc = time_candidates(c, shapes, trial_steps, &ms2, &b2);
@@ -1245,7 +1302,7 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   // confirmation: the winner against the library default (its layout resolved)
   // in trials three times as long; a winner that is not faster there is not
   // kept, since a short trial can favour a candidate by chance
-  Candidate dflt{0, -1, 1, c->nblocks >= 2048 ? 32 : 0, 0};
+  Candidate dflt{0, -1, 1, c->nblocks >= 2048 ? 32 : 0, 0, 1, 0};
   rc = apply_candidate(c, dflt);
   if (rc) return rc;
   dflt.group_rows = c->p.pop_group_rows;
@@ -1268,6 +1325,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
   out->cells_per_thread = best.cpt;
   out->xcd_remap = best.remap;
   out->persistent_wg_per_cu = best.pers;
+  out->steps_per_pass = best.spp;
+  out->pass_planes = best.planes;
   out->candidates = (int32_t)(lays.size() + shapes.size() + fin.size());
   out->ms_per_step = ms3[b3];
   out->ms_per_step_default = ms3[1];

@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
         ("xcd_remap", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("f16_rounding", ctypes.c_int32),
         ("pop_group_rows", ctypes.c_int32), ("pop_pad_bytes", ctypes.c_int32),
         ("persistent_wg_per_cu", ctypes.c_int32), ("collision_model", ctypes.c_int32),
-        ("rr_order", ctypes.c_int32),
+        ("rr_order", ctypes.c_int32), ("steps_per_pass", ctypes.c_int32), ("pass_planes", ctypes.c_int32),
     ]
 
 
@@ -68,6 +68,7 @@ class Tuning(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
         ("persistent_wg_per_cu", ctypes.c_int32), ("candidates", ctypes.c_int32),
         ("ms_per_step", ctypes.c_double), ("ms_per_step_default", ctypes.c_double),
+        ("steps_per_pass", ctypes.c_int32), ("pass_planes", ctypes.c_int32),
     ]
 
     def as_dict(self):
