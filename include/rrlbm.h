@@ -254,8 +254,10 @@ int rrlbm_fused_step_gp(rrlbm_ctx* ctx, int istep, double rand_phase);
 int rrlbm_fused_step_timed(rrlbm_ctx* ctx, int nsteps, double* ms);
 /* Capture nsteps fused steps into a HIP graph (replaces a previous capture);
  * the reference launches every kernel from the host each step (main.cu:31-117).
- * Replays must start from the buffer the capture started from: with an odd
- * nsteps every second replay is refused (RRLBM_E_STATE; capture again).  With
+ * Replays must start from the buffer the capture started from: when the
+ * captured launches swap the buffers an odd number of times (an odd nsteps; with
+ * steps_per_pass 2 one swap per two steps, so nsteps = 2 mod 4 as well) every
+ * second replay is refused (RRLBM_E_STATE; capture again).  With
  * FP16 stochastic rounding the captured steps read their step number from a
  * device counter of the context that each replay advances, so replays give
  * the state of as many eager steps, dither included.  A change of population

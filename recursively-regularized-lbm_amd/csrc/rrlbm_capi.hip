@@ -60,6 +60,8 @@ struct rrlbm_ctx {
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
   bool graph_dev_step = false;  // the captured steps read d_step (advanced by every replay)
+  bool graph_swap_pop = false;  // a replay leaves the state in the other population buffer
+  bool graph_swap_u = false;    // ... and the lagged LES velocity in the other velocity buffer
   void* graph_pop_a = nullptr;  // buffer the captured sequence starts from
   Fused2Args pass{};   // steps_per_pass 2: tiles and z chunks of the two-step launch
   bool ref_ok = true;  // reference-layout fields allocated
@@ -355,6 +357,8 @@ void drop_graph(rrlbm_ctx* c) {
   c->graph_steps = 0;
   c->graph_pop_a = nullptr;
   c->graph_dev_step = false;
+  c->graph_swap_pop = false;
+  c->graph_swap_u = false;
 }
 
 bool f16_dither(const rrlbm_params& p) {
@@ -916,6 +920,9 @@ int rrlbm_graph_capture(rrlbm_ctx* c, int nsteps) {
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (e == hipSuccess) e = e2;
+  // the buffer swaps of the captured launches (one per launch: a two-step
+  // launch advances two steps with one swap)
+  const bool swap_pop = c->pop_a != a0, swap_u = c->u_a != ua0;
   c->pop_a = a0; c->pop_b = b0; c->u_a = ua0; c->u_b = ub0; c->sr_step = sr0;
   c->capture_dev_step = false;
   if (rc || e != hipSuccess) {
@@ -928,6 +935,8 @@ int rrlbm_graph_capture(rrlbm_ctx* c, int nsteps) {
   c->graph_steps = nsteps;
   c->graph_pop_a = a0;
   c->graph_dev_step = dev_step;
+  c->graph_swap_pop = swap_pop;
+  c->graph_swap_u = swap_u;
   return RRLBM_OK;
 }
 
@@ -945,12 +954,10 @@ int rrlbm_graph_launch(rrlbm_ctx* c, int times) {
   }
   for (int t = 0; t < times; ++t) {
     if (c->pop_a != c->graph_pop_a)
-      return fail(c, RRLBM_E_STATE, "the state sits in the other buffer than at capture (odd step count); capture again");
+      return fail(c, RRLBM_E_STATE, "the state sits in the other buffer than at capture (an odd number of launches was captured); capture again");
     CK(c, hipGraphLaunch(c->exec, c->stream));
-    for (int s = 0; s < c->graph_steps; ++s) {
-      std::swap(c->pop_a, c->pop_b);
-      if (c->p.use_les && c->p.les_lagged) std::swap(c->u_a, c->u_b);
-    }
+    if (c->graph_swap_pop) std::swap(c->pop_a, c->pop_b);
+    if (c->graph_swap_u) std::swap(c->u_a, c->u_b);
     c->sr_step += (uint32_t)c->graph_steps;
     c->force_cur = rrlbm_ctx::Forcing{};
   }
@@ -1360,6 +1367,8 @@ struct rrlbm_batch {
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
   bool graph_dev_step = false;  // the captured steps read d_step (advanced by every replay)
+  bool graph_swap_pop = false;  // a replay leaves the state in the other population buffer
+  bool graph_swap_u = false;    // ... and the lagged LES velocity in the other velocity buffer
   uint32_t graph_parity = 0;
   int64_t cells = 0;
   std::string err;

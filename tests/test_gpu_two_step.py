@@ -70,9 +70,13 @@ def test_two_step_fp64_matches_oracle(rr, name):
         assert maxabs(a, b) < 1e-12
 
 
-def test_two_step_graph_replays_equal_eager_steps(rr):
-    """A captured graph of four steps (two two-step launches), replayed three
-    times, equals twelve eager steps."""
+@pytest.mark.parametrize("nsteps,times", [(4, 3), (2, 1), (6, 1), (5, 1)])
+def test_two_step_graph_replays_equal_eager_steps(rr, nsteps, times):
+    """A captured graph of nsteps steps (two-step launches, and a one-step
+    launch for an odd count), replayed `times` times, equals as many eager
+    steps followed by one more; a graph whose launches swap the buffers an odd
+    number of times (2 or 6 steps: 1 or 3 launches; 5 steps: 3) refuses a
+    second replay."""
     g, dims, params, force, walls, lid = state("channel")
     kw = dict(lx=g.lx, ly=g.ly, lz=g.lz, npop=27, storage=1, tau=0.53, force=force, steps_per_pass=2, **params)
     out = []
@@ -81,10 +85,15 @@ def test_two_step_graph_replays_equal_eager_steps(rr):
             upload_state(s, g)
             s.fused_pack()
             if graph:
-                s.graph_capture(4)
-                s.graph_launch(3)
+                s.graph_capture(nsteps)
+                s.graph_launch(times)
+                if nsteps % 4 != 0:
+                    with pytest.raises(rr.RRLBMError) as ei:
+                        s.graph_launch(1)
+                    assert ei.value.status == 3
             else:
-                s.fused_step(12)
+                s.fused_step(nsteps * times)
+            s.fused_step(1)
             s.fused_unpack()
             out.append(s.download(rr.Field.F))
     assert np.array_equal(out[0], out[1]), maxabs(out[0], out[1])
