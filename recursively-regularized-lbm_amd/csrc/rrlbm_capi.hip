@@ -304,18 +304,18 @@ int launch_step(rrlbm_ctx* c, const FusedArgs& a) {
 
 // next: the force of the collided steps (nullptr: the uniform force)
 int fused_steps(rrlbm_ctx* c, int nsteps, const rrlbm_ctx::Forcing* next = nullptr) {
-  int s = 0;
-  if (c->p.steps_per_pass == 2 && !next && !c->force_cur.gp) {
-    // pairs of steps in one launch each (uniform force throughout)
-    for (; s + 2 <= nsteps; s += 2) {
+  for (int s = 0; s < nsteps; ++s) {
+    if (c->p.steps_per_pass == 2 && !next && !c->force_cur.gp && s + 2 <= nsteps) {
+      // two steps in one launch (the step it completes and both it collides
+      // take the uniform force)
       const FusedArgs a = fused_args(c);
       hipError_t e = launch_fused2(c->p.storage, has_walls(c->p), a, c->pass, c->stream);
       if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "two-step kernel launch failed: %s", hipGetErrorString(e));
       c->sr_step += 2;
       std::swap(c->pop_a, c->pop_b);
+      ++s;
+      continue;
     }
-  }
-  for (; s < nsteps; ++s) {
     FusedArgs a = fused_args(c);
     if (next && next->gp) {
       a.gp_on[1] = 1;

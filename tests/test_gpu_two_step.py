@@ -120,3 +120,23 @@ def test_autotune_times_two_step_launches(rr):
             s.fused_unpack()
             out.append(s.download(rr.Field.F))
     assert maxabs(out[0], out[1]) < 1e-6
+
+
+def test_two_step_after_forcing_gp_steps(rr):
+    """forcing_GP steps run as one-step launches; the plain steps after them
+    (one single step to complete the GP collision, then pairs) equal one-step
+    launches of the same sequence."""
+    g, dims, params, force, walls, lid = state("channel")
+    out = []
+    for spp in (1, 2):
+        with rr.Solver(lx=g.lx, ly=g.ly, lz=g.lz, npop=27, storage=1, tau=0.53, force=force, steps_per_pass=spp,
+                       **params) as s:
+            upload_state(s, g)
+            s.fused_pack()
+            s.fused_step(3)
+            s.fused_step_gp(40, 0.3)
+            s.fused_step_gp(41, 0.3)
+            s.fused_step(5)
+            s.fused_unpack()
+            out.append(s.download(rr.Field.F))
+    assert np.array_equal(out[0], out[1]), maxabs(out[0], out[1])
