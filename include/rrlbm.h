@@ -134,7 +134,7 @@ typedef struct rrlbm_params {
                                    storage with lx a multiple of 64 and ly of 4; an odd step count
                                    and forcing_GP steps take single launches */
     int32_t pass_planes;        /* steps_per_pass 2: z planes per workgroup's chunk (0 = library
-                                   default: at least six chunks per compute unit) */
+                                   default, 16) */
 } rrlbm_params;
 
 typedef struct rrlbm_ctx rrlbm_ctx;
@@ -293,7 +293,7 @@ typedef struct rrlbm_tuning {
  * padding) and launch shapes (1 or 2 cells per thread, block order over the
  * XCDs: linear, slabs, runs of 4, 16, 32 or 64 blocks; one workgroup per block or a
  * persistent grid of 4 or 8 per compute unit; where the configuration allows, two
- * steps per launch with z chunks of 8, 16 or 24 planes), best of three interleaved
+ * steps per launch with z chunks of 8, 12, 16, 24 or 32 planes), best of three interleaved
  * rounds of trial_steps steps each; then times the fastest against the
  * library default in rounds of 3 * trial_steps and keeps the default unless
  * the fastest still beats it.  The choice applies to ctx and is cached per

@@ -1049,10 +1049,184 @@ fused2_kernel(FusedArgs a, Fused2Args t) {
   }
 }
 
+// FP32: the two steps overlap in different wavefronts of the workgroup.
+// Wavefronts 0-6 compute step t+1 at plane k (the six halo rows and the two
+// halo columns, as above) and write all its populations into the ring;
+// wavefronts 7-10 compute step t+2 at plane k - 2 from planes k - 3 .. k - 1
+// of the ring, which the barrier that ends each plane has made visible.  The
+// ring keeps the c_z = +1 populations of four planes and the c_z = 0 and -1
+// populations of three (90 x 396 values, 142.6 KB: one workgroup of 11
+// wavefronts per compute unit).  The step-t+1 wavefronts issue the gather of
+// the next plane before the barrier.
+constexpr uint32_t kT2PThreads = 11 * 64;
+
+template <bool WALLS, int FM>
+__global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))) void fused2p_kernel(
+    FusedArgs a, Fused2Args t) {
+  constexpr int S = kF32;
+  using C = float;
+  using T = float;
+  using L = D3Q27;
+  constexpr int Q = 27;
+  constexpr uint32_t kH2Y = kT2Y + 2, kH2N = kH2X * kH2Y;
+  const uint32_t f = (blockIdx.x & 7u) * t.per_xcd + (blockIdx.x >> 3);
+  if (f >= t.items) return;  // the whole workgroup: no barrier is left half-reached
+  const uint32_t tx = f % t.ntx, ty = (f / t.ntx) % t.nty, ch = f / (t.nty * t.ntx);
+  const uint32_t x0 = tx * kT2X, y0 = ty * kT2Y, z0 = ch * t.zc;
+  const uint32_t zlen = min(t.zc, a.lz - z0);
+
+  __shared__ C sP[4][9][kH2N], sZ[3][9][kH2N], sM[3][9][kH2N];
+
+  const uint32_t tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
+  const bool step1 = w <= kH2Y;  // wavefronts 0-6
+  // step t+1 cell (wavefronts 0-5: halo row w, columns 1..64; wavefront 6
+  // lanes 0-11: the halo columns 0 and 65)
+  uint32_t hx, hy;
+  bool h_on = step1;
+  if (w < kH2Y) {
+    hy = w;
+    hx = lane + 1;
+  } else {
+    h_on = step1 && lane < 2 * kH2Y;
+    hy = h_on ? lane >> 1 : 0u;
+    hx = (lane & 1u) ? kH2X - 1 : 0u;
+  }
+  const uint32_t hidx = hy * kH2X + hx;
+  const uint32_t gx = (x0 + hx + a.lx - 1) % a.lx, gy = (y0 + hy + a.ly - 1) % a.ly;
+  // step t+2 cell (wavefronts 7-10: tile row r, column lane)
+  const uint32_t r = step1 ? 0u : w - (kH2Y + 1);
+  const uint32_t iidx = (r + 1) * kH2X + lane + 1;
+  const uint32_t ix = x0 + lane, iy = y0 + r;
+
+  const C Fx = C(a.fx), Fy = C(a.fy), Fz = C(a.fz);
+  const C omega = C(a.omega);
+  constexpr uint32_t sz = sizeof(T);
+  const uint32_t row_b = a.lx * sz;
+  auto on_wall = [&](uint32_t x, uint32_t y, uint32_t z) {
+    return ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
+           ((a.walls & 4) && (z == 0 || z == a.lz - 1));
+  };
+  auto collide = [&](C (&h)[Q], C (&out)[Q]) {
+    const Moments<C> m = moments_q27<C>(h);
+    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
+    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+  };
+  const uint32_t gyr = (y0 + w + a.ly - 1) % a.ly;  // = gy in the halo-row wavefronts, uniform
+  const uint32_t ysr[3] = {gyr + 1 == a.ly ? 0u : gyr + 1, gyr, gyr == 0 ? a.ly - 1 : gyr - 1};
+  const uint32_t xbr[3] = {(gx + 1 == a.lx ? 0u : gx + 1) * sz, gx * sz, (gx == 0 ? a.lx - 1 : gx - 1) * sz};
+  auto row_gather = [&](uint32_t kk, C (&dst)[Q]) {
+    const uint32_t zs[3] = {kk + 1 == a.lz ? 0u : kk + 1, kk, kk == 0 ? a.lz - 1 : kk - 1};
+    uint32_t ro[3][3];
+    static_for<3>([&](auto jy) {
+      static_for<3>([&](auto jz) {
+        constexpr int iy2 = decltype(jy)::value, iz2 = decltype(jz)::value;
+        ro[iy2][iz2] = row_offset32(a, row_b, ysr[iy2], zs[iz2]);
+      });
+    });
+    uint64_t pb = reinterpret_cast<uint64_t>(a.src);
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xbr[cx + 1]));
+      pb += a.lay.seg;
+      asm volatile("" : "+s"(pb));
+    });
+  };
+  C hn[Q];
+  if (w < kH2Y) row_gather((z0 + a.lz - 1) % a.lz, hn);
+
+  // iteration it: step t+1 at plane z0 - 1 + it (it <= zlen + 1), step t+2 at
+  // plane z0 - 3 + it (it >= 3); ring slots of the plane of iteration i:
+  // sP[i % 4], sZ[i % 3], sM[i % 3]
+  for (uint32_t it = 0; it < zlen + 3; ++it) {
+    const uint32_t p4 = it & 3u, p3 = it % 3u;
+    if (step1) {
+      if (it < zlen + 2 && h_on) {
+        const uint32_t k = (z0 + a.lz - 1 + it) % a.lz;
+        C h[Q];
+        if (w < kH2Y)
+          static_for<Q>([&](auto ic) { h[decltype(ic)::value] = hn[decltype(ic)::value]; });
+        else
+          gather_lane<S, L>(a, gx, gy, k, h);
+        if constexpr (WALLS) {
+          if (on_wall(gx, gy, k)) {
+            const uint32_t off = cell_off<S>(a.lay, (k * a.ly + gy) * a.lx + gx);
+            wall_treatment<S, L, FM>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
+          }
+        }
+        C o1[Q];
+        collide(h, o1);
+        static_for<Q>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          constexpr int j = zgroup_index<L>(i);
+          if constexpr (L::cz(i) > 0)
+            sP[p4][j][hidx] = o1[i];
+          else if constexpr (L::cz(i) == 0)
+            sZ[p3][j][hidx] = o1[i];
+          else
+            sM[p3][j][hidx] = o1[i];
+        });
+        if (w < kH2Y && it + 1 < zlen + 2) {
+          const uint32_t kn = k + 1 == a.lz ? 0u : k + 1;
+          row_gather(kn, hn);
+        }
+      }
+    } else if (it >= 3) {
+      // step t+2 at plane kz = z0 - 3 + it; planes kz - 1, kz, kz + 1 were
+      // written in iterations it - 3, it - 2, it - 1
+      const uint32_t kz = (z0 + it - 3) % a.lz;
+      const uint32_t pm3 = (it + 1) & 3u, pm2 = (it + 2) & 3u;  // P slots of planes kz - 1, kz
+      const uint32_t qm2 = (it + 1) % 3u, qm1 = (it + 2) % 3u;  // Z / M slots of planes kz, kz + 1
+      C h[Q];
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i), j = zgroup_index<L>(i);
+        const uint32_t src = iidx - (uint32_t)(cy * (int)kH2X + cx);
+        if constexpr (cz > 0)
+          h[i] = sP[pm3][j][src];
+        else if constexpr (cz == 0)
+          h[i] = sZ[qm2][j][src];
+        else
+          h[i] = sM[qm1][j][src];
+      });
+      if constexpr (WALLS) {
+        if (on_wall(ix, iy, kz)) {
+          wall_treatment_own<L, C, FM, false>(
+              a, h, ix, iy, kz,
+              [&](int o) -> C {
+                const int j = (L::cx(o) + 1) * 3 + (L::cy(o) + 1);
+                return L::cz(o) > 0 ? sP[pm2][j][iidx] : (L::cz(o) == 0 ? sZ[qm2][j][iidx] : sM[qm2][j][iidx]);
+              },
+              Fx, Fy, Fz);
+        }
+      }
+      C o2[Q];
+      collide(h, o2);
+      char* own_dst = reinterpret_cast<char*>(a.dst) + row_offset(a, row_b, iy, kz);
+      store_cell<S, Q>(a, own_dst, ix * sz, (kz * a.ly + iy) * a.lx + ix, o2);
+    }
+    __syncthreads();
+  }
+}
+
+template <bool W>
+hipError_t launch_fused2p(int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  const dim3 grid(8u * t.per_xcd), block(kT2PThreads);
+  switch (fm) {
+    case kForceNone: hipLaunchKernelGGL((fused2p_kernel<W, kForceNone>), grid, block, 0, s, a, t); break;
+    case kForceX: hipLaunchKernelGGL((fused2p_kernel<W, kForceX>), grid, block, 0, s, a, t); break;
+    default: hipLaunchKernelGGL((fused2p_kernel<W, kForceAll>), grid, block, 0, s, a, t); break;
+  }
+  return hipGetLastError();
+}
+
 template <int S>
 hipError_t launch_fused2_storage(bool walls, int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
   if constexpr (S == kF16) {
     return hipErrorInvalidValue;
+  } else if constexpr (S == kF32) {
+    return walls ? launch_fused2p<true>(fm, a, t, s) : launch_fused2p<false>(fm, a, t, s);
   } else {
     const dim3 grid(8u * t.per_xcd), block(kT2Threads);
     if (walls) {

@@ -234,19 +234,15 @@ bool two_step_ok(const rrlbm_params& p) {
 }
 
 // Geometry of the two-step launch: tiles of kT2X x kT2Y cells, z chunks of
-// pass_planes planes (default: the fewest chunks that give at least six work
-// items per compute unit), the items dealt to the 8 XCDs in contiguous runs.
-Fused2Args pass_geometry(const rrlbm_params& p, int cus) {
+// pass_planes planes (default 16: on the 384 x 128 x 192 channel the fastest of
+// 8-48, nine work items per compute unit), the items dealt to the 8 XCDs in
+// contiguous runs.
+Fused2Args pass_geometry(const rrlbm_params& p) {
   Fused2Args t{};
   t.ntx = (uint32_t)p.lx / kT2X;
   t.nty = (uint32_t)p.ly / kT2Y;
   const uint32_t tiles = t.ntx * t.nty, lz = (uint32_t)p.lz;
-  uint32_t zc = (uint32_t)p.pass_planes;
-  if (zc == 0) {
-    const uint32_t want = (6u * (uint32_t)cus + tiles - 1) / tiles;
-    const uint32_t nzc = std::max(1u, std::min(lz, want));
-    zc = (lz + nzc - 1) / nzc;
-  }
+  const uint32_t zc = p.pass_planes > 0 ? (uint32_t)p.pass_planes : 16u;
   t.zc = std::min(zc, lz);
   t.nzc = (lz + t.zc - 1) / t.zc;
   t.items = tiles * t.nzc;
@@ -572,7 +568,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) return bail(fail(c, RRLBM_E_HIP, "HIP setup failed: %s", hipGetErrorString(e)));
-  if (p.steps_per_pass == 2) c->pass = pass_geometry(p, c->cus);
+  if (p.steps_per_pass == 2) c->pass = pass_geometry(p);
 
   const size_t n = c->n, qn = (size_t)c->q * n;
   int rc = RRLBM_OK;
@@ -1203,7 +1199,7 @@ int apply_candidate(rrlbm_ctx* c, const Candidate& k) {
   if (c->p.steps_per_pass != k.spp || c->p.pass_planes != k.planes) drop_graph(c);
   c->p.steps_per_pass = k.spp;
   c->p.pass_planes = k.planes;
-  if (k.spp == 2) c->pass = pass_geometry(c->p, c->cus);
+  if (k.spp == 2) c->pass = pass_geometry(c->p);
   return RRLBM_OK;
 }
 
@@ -1298,9 +1294,9 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
         if (cpt == 2 && !pair_ok) continue;
         shapes.push_back({base.group_rows, base.pad, cpt, remap, pers});
       }
-  // two steps per launch, z chunks of 8, 16 or 24 planes
+  // two steps per launch, z chunks of 8 to 32 planes
   if (two_step_ok(c->p))
-    for (int planes : {8, 16, 24})
+    for (int planes : {8, 12, 16, 24, 32})
       if (planes <= c->p.lz) shapes.push_back({base.group_rows, base.pad, 1, base.remap, 0, 2, planes});
   std::vector<double> ms2;
   int b2 = 0;
