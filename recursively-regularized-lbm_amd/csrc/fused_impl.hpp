@@ -858,7 +858,8 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
 // The stored state of step t (post-collision populations in a.src) goes to
 // that of step t + 2 (a.dst) in one pass, so each population crosses HBM once
 // per two steps.  A workgroup owns a tile of kT2X x kT2Y cells in (x, y) and a
-// chunk of planes [z0, z1) in z, and marches k = z0 - 1 .. z1:
+// chunk of planes [z0, z1) in z, and marches k = z0 - 1 .. z1.  In FP64
+// (fused2_kernel), per plane:
 //   A. step t+1 at plane k on the tile and a one-cell halo around it
 //      ((kT2X + 2) x (kT2Y + 2) cells), gathered from a.src as the one-step
 //      kernel does, into registers; the populations with c_z = -1 go to LDS;
@@ -867,10 +868,12 @@ __global__ __launch_bounds__(kBlock) void fused_pair_kernel(FusedArgs a) {
 //   C. the populations of plane k with c_z = 0 and +1 go to LDS.
 // The ring holds plane k - 1 in full while B runs (the wall cells' bounce-back
 // reads their own populations of that plane): c_z = -1 and +1 in two slots
-// each, c_z = 0 in one, 45 x 396 values.  Both steps call the one-step
-// kernel's per-cell functions, so the result equals two one-step launches bit
-// for bit.  Planes k - 1 and z1 of step t+1 are computed by both chunks that
-// border them.
+// each, c_z = 0 in one, 45 x 396 values.  FP32 (fused2p_kernel) overlaps the
+// two steps in different wavefronts.  Both steps call the one-step kernel's
+// per-cell functions, so the result equals two one-step launches: bit for bit
+// where the compiler contracts the same multiply-adds in both kernels, to an
+// ulp where it does not.  Planes z0 - 1 and z1 of step t+1 are computed by
+// both chunks that border them.
 constexpr uint32_t kH2X = kT2X + 2, kH2Y = kT2Y + 2, kH2N = kH2X * kH2Y;
 
 // Index of population i among the nine with the same c_z.
@@ -879,19 +882,16 @@ RRLBM_HD constexpr int zgroup_index(int i) {
   return (L::cx(i) + 1) * 3 + (L::cy(i) + 1);
 }
 
-// Occupancy: FP64 (142 KB of LDS) runs one workgroup per compute unit.  FP32
-// (71 KB) prefetches the next plane's step-1 gather into registers while step
-// 2 runs, which needs more registers than two workgroups of the walled kernel
-// allow (one workgroup, 156 VGPRs; measured faster than two without the
-// prefetch); the periodic kernel keeps two (4 wavefronts per SIMD).
+// FP64 (the FP32 form is fused2p_kernel below): 142.6 KB of LDS, one
+// workgroup of 7 wavefronts per compute unit.
 template <int S, bool WALLS, int FM>
-__global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu((S == kF32 && !WALLS) ? 4 : 2))) void
-fused2_kernel(FusedArgs a, Fused2Args t) {
+__global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2))) void fused2_kernel(FusedArgs a,
+                                                                                                   Fused2Args t) {
   using C = typename StoreT<S>::C;
   using T = typename StoreT<S>::T;
   using L = D3Q27;
   constexpr int Q = 27;
-  static_assert(S != kF16, "two steps per launch: FP32 or FP64 storage");
+  static_assert(S == kF64, "two steps per launch in FP64 storage (FP32: fused2p_kernel)");
   // work item: XCD (blockIdx.x mod 8) takes a contiguous run of items, x
   // fastest, so the workgroups an XCD runs at one time cover whole rows of
   // tiles and share their halo cache lines in its L2
@@ -944,8 +944,6 @@ fused2_kernel(FusedArgs a, Fused2Args t) {
 
   // Halo rows: (y, z) is uniform across the wavefront, so the nine source rows
   // of the step-1 gather are scalar bases and each lane adds its x offset.
-  // FP32 issues the gather of the next plane before step 2 of this one.
-  constexpr bool kPrefetch = S == kF32;
   const uint32_t gyr = (y0 + w + a.ly - 1) % a.ly;  // = gy in the halo-row wavefronts, uniform
   const uint32_t ysr[3] = {gyr + 1 == a.ly ? 0u : gyr + 1, gyr, gyr == 0 ? a.ly - 1 : gyr - 1};
   const uint32_t xbr[3] = {(gx + 1 == a.lx ? 0u : gx + 1) * sz, gx * sz, (gx == 0 ? a.lx - 1 : gx - 1) * sz};
@@ -967,10 +965,6 @@ fused2_kernel(FusedArgs a, Fused2Args t) {
       asm volatile("" : "+s"(pb));
     });
   };
-  [[maybe_unused]] C hn[Q];
-  if constexpr (kPrefetch) {
-    if (w < kH2Y) row_gather((z0 + a.lz - 1) % a.lz, hn);
-  }
 
   for (uint32_t it = 0; it < zlen + 2; ++it) {
     const uint32_t k = (z0 + a.lz - 1 + it) % a.lz;  // plane of step t+1
@@ -980,10 +974,7 @@ fused2_kernel(FusedArgs a, Fused2Args t) {
     if (h_on) {
       C h[Q];
       if (w < kH2Y) {
-        if constexpr (kPrefetch)
-          static_for<Q>([&](auto ic) { h[decltype(ic)::value] = hn[decltype(ic)::value]; });
-        else
-          row_gather(k, h);
+        row_gather(k, h);
       } else {
         gather_lane<S, L>(a, gx, gy, k, h);
       }
@@ -998,9 +989,6 @@ fused2_kernel(FusedArgs a, Fused2Args t) {
         constexpr int i = decltype(ic)::value;
         if constexpr (L::cz(i) < 0) sM[p][zgroup_index<L>(i)][hidx] = o1[i];
       });
-    }
-    if constexpr (kPrefetch) {
-      if (w < kH2Y && it + 1 < zlen + 2) row_gather(k + 1 == a.lz ? 0u : k + 1, hn);
     }
     __syncthreads();
     // B. step t+2 at plane k - 1
@@ -1227,7 +1215,7 @@ hipError_t launch_fused2_storage(bool walls, int fm, const FusedArgs& a, const F
     return hipErrorInvalidValue;
   } else if constexpr (S == kF32) {
     return walls ? launch_fused2p<true>(fm, a, t, s) : launch_fused2p<false>(fm, a, t, s);
-  } else {
+  } else {  // FP64
     const dim3 grid(8u * t.per_xcd), block(kT2Threads);
     if (walls) {
       switch (fm) {
