@@ -956,11 +956,16 @@ __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2)))
         ro[iy2][iz2] = row_offset32(a, row_b, ysr[iy2], zs[iz2]);
       });
     });
+    // the lane's x offsets are made opaque per call: a zero extension of them
+    // hoisted out of the z loop would make every load a 64-bit per-lane
+    // address add instead of a scalar base plus a 32-bit lane offset
+    uint32_t xo[3] = {xbr[0], xbr[1], xbr[2]};
+    asm volatile("" : "+v"(xo[0]), "+v"(xo[1]), "+v"(xo[2]));
     uint64_t pb = reinterpret_cast<uint64_t>(a.src);
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xbr[cx + 1]));
+      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xo[cx + 1]));
       pb += a.lay.seg;
       asm volatile("" : "+s"(pb));
     });
@@ -1112,11 +1117,16 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
         ro[iy2][iz2] = row_offset32(a, row_b, ysr[iy2], zs[iz2]);
       });
     });
+    // the lane's x offsets are made opaque per call: a zero extension of them
+    // hoisted out of the z loop would make every load a 64-bit per-lane
+    // address add instead of a scalar base plus a 32-bit lane offset
+    uint32_t xo[3] = {xbr[0], xbr[1], xbr[2]};
+    asm volatile("" : "+v"(xo[0]), "+v"(xo[1]), "+v"(xo[2]));
     uint64_t pb = reinterpret_cast<uint64_t>(a.src);
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xbr[cx + 1]));
+      dst[i] = C(*reinterpret_cast<const GlobalT<T>*>(pb + ro[cy + 1][cz + 1] + xo[cx + 1]));
       pb += a.lay.seg;
       asm volatile("" : "+s"(pb));
     });
