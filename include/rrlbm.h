@@ -130,9 +130,9 @@ typedef struct rrlbm_params {
                                    112-116; BC.cu:37-38,50-54,79-88,97-101); 6: D3Q27, RR, no LES */
     int32_t steps_per_pass;     /* fused steps per launch: 0 or 1 = one; 2 = two steps in one pass
                                    over the populations (the first step's state stays on chip), for
-                                   the D3Q27 third-order RR step without LES in FP32 or FP64
-                                   storage with lx a multiple of 64 and ly of 4; an odd step count
-                                   and forcing_GP steps take single launches */
+                                   the D3Q27 RR (orders 3 and 6) and BGK steps without LES or FDBC
+                                   in FP32 or FP64 storage with lx a multiple of 64 and ly of 4;
+                                   an odd step count and forcing_GP steps take single launches */
     int32_t pass_planes;        /* steps_per_pass 2: z planes per workgroup's chunk (0 = library
                                    default, 16) */
 } rrlbm_params;

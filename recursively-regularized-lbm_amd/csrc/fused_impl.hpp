@@ -882,9 +882,32 @@ RRLBM_HD constexpr int zgroup_index(int i) {
   return (L::cx(i) + 1) * 3 + (L::cy(i) + 1);
 }
 
+// The collision of a cell in the two-step kernels, whose pulled (and wall-
+// treated) populations are h: OP = kOpRR the RR collision, kOpBGK the BGK
+// operator with the BGK solver's velocity j + F/2, kOpRR6 the RR collision
+// with the fourth- to sixth-order terms; the per-cell functions and force
+// forms of the one-step kernel's collision.
+template <typename C, int FM, int OP>
+RRLBM_INLINE __device__ void collide_cell(const C (&h)[27], C Fx, C Fy, C Fz, C omega, C (&out)[27]) {
+  const Moments<C> m = moments_q27<C>(h);
+  if constexpr (OP == kOpBGK) {
+    const C inv_tau = C(1) - omega;
+    const C fcoef = C(1) - C(0.5) * inv_tau;
+    bgk_post_q27<C, true>(h, m.dr, m.jx + C(0.5) * Fx, m.jy + C(0.5) * Fy, m.jz + C(0.5) * Fz, Fx, Fy, Fz, inv_tau,
+                          fcoef, C(1), out);
+  } else {
+    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
+    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
+    if constexpr (OP == kOpRR6)
+      rr_post_q27_o6<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+    else
+      rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
+  }
+}
+
 // FP64 (the FP32 form is fused2p_kernel below): 142.6 KB of LDS, one
 // workgroup of 7 wavefronts per compute unit.
-template <int S, bool WALLS, int FM>
+template <int S, bool WALLS, int FM, int OP>
 __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2))) void fused2_kernel(FusedArgs a,
                                                                                                    Fused2Args t) {
   using C = typename StoreT<S>::C;
@@ -933,14 +956,7 @@ __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2)))
     return ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
            ((a.walls & 4) && (z == 0 || z == a.lz - 1));
   };
-  // moments, macrovar and the RR collision of a cell whose pulled (and wall-
-  // treated) populations are h
-  auto collide = [&](C (&h)[Q], C (&out)[Q]) {
-    const Moments<C> m = moments_q27<C>(h);
-    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
-    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
-    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
-  };
+  auto collide = [&](C (&h)[Q], C (&out)[Q]) { collide_cell<C, FM, OP>(h, Fx, Fy, Fz, omega, out); };
 
   // Halo rows: (y, z) is uniform across the wavefront, so the nine source rows
   // of the step-1 gather are scalar bases and each lane adds its x offset.
@@ -986,7 +1002,7 @@ __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2)))
       if constexpr (WALLS) {
         if (on_wall(gx, gy, k)) {
           const uint32_t off = cell_off<S>(a.lay, (k * a.ly + gy) * a.lx + gx);
-          wall_treatment<S, L, FM>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
+          wall_treatment<S, L, FM, OP == kOpRR6>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
         }
       }
       collide(h, o1);
@@ -1014,7 +1030,7 @@ __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2)))
       });
       if constexpr (WALLS) {
         if (on_wall(ix, iy, kz)) {
-          wall_treatment_own<L, C, FM, false>(
+          wall_treatment_own<L, C, FM, OP == kOpRR6>(
               a, h, ix, iy, kz,
               [&](int o) -> C {
                 const int j = (L::cx(o) + 1) * 3 + (L::cy(o) + 1);
@@ -1053,7 +1069,7 @@ __global__ __launch_bounds__(kT2Threads) __attribute__((amdgpu_waves_per_eu(2)))
 // the next plane before the barrier.
 constexpr uint32_t kT2PThreads = 11 * 64;
 
-template <bool WALLS, int FM>
+template <bool WALLS, int FM, int OP>
 __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))) void fused2p_kernel(
     FusedArgs a, Fused2Args t) {
   constexpr int S = kF32;
@@ -1099,12 +1115,7 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
     return ((a.walls & 1) && (x == 0 || x == a.lx - 1)) || ((a.walls & 2) && (y == 0 || y == a.ly - 1)) ||
            ((a.walls & 4) && (z == 0 || z == a.lz - 1));
   };
-  auto collide = [&](C (&h)[Q], C (&out)[Q]) {
-    const Moments<C> m = moments_q27<C>(h);
-    const Macro<C> q = macro_of<C, FM>(m, Fx, Fy, Fz);
-    const NeqTensor<C> N = neq_q27<C, FM>(m, q, Fx, Fy, Fz);
-    rr_post_q27<C, true, FM>(q, N, Fx, Fy, Fz, omega, C(1), C(1), out);
-  };
+  auto collide = [&](C (&h)[Q], C (&out)[Q]) { collide_cell<C, FM, OP>(h, Fx, Fy, Fz, omega, out); };
   const uint32_t gyr = (y0 + w + a.ly - 1) % a.ly;  // = gy in the halo-row wavefronts, uniform
   const uint32_t ysr[3] = {gyr + 1 == a.ly ? 0u : gyr + 1, gyr, gyr == 0 ? a.ly - 1 : gyr - 1};
   const uint32_t xbr[3] = {(gx + 1 == a.lx ? 0u : gx + 1) * sz, gx * sz, (gx == 0 ? a.lx - 1 : gx - 1) * sz};
@@ -1150,7 +1161,7 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
         if constexpr (WALLS) {
           if (on_wall(gx, gy, k)) {
             const uint32_t off = cell_off<S>(a.lay, (k * a.ly + gy) * a.lx + gx);
-            wall_treatment<S, L, FM>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
+            wall_treatment<S, L, FM, OP == kOpRR6>(a, h, gx, gy, k, a.src, off, Fx, Fy, Fz);
           }
         }
         C o1[Q];
@@ -1190,7 +1201,7 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
       });
       if constexpr (WALLS) {
         if (on_wall(ix, iy, kz)) {
-          wall_treatment_own<L, C, FM, false>(
+          wall_treatment_own<L, C, FM, OP == kOpRR6>(
               a, h, ix, iy, kz,
               [&](int o) -> C {
                 const int j = (L::cx(o) + 1) * 3 + (L::cy(o) + 1);
@@ -1208,39 +1219,54 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
   }
 }
 
-template <bool W>
-hipError_t launch_fused2p(int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
-  const dim3 grid(8u * t.per_xcd), block(kT2PThreads);
-  switch (fm) {
-    case kForceNone: hipLaunchKernelGGL((fused2p_kernel<W, kForceNone>), grid, block, 0, s, a, t); break;
-    case kForceX: hipLaunchKernelGGL((fused2p_kernel<W, kForceX>), grid, block, 0, s, a, t); break;
-    default: hipLaunchKernelGGL((fused2p_kernel<W, kForceAll>), grid, block, 0, s, a, t); break;
+// Two-step launch of one storage type, walls flag and operator; the force
+// forms are those of the one-step kernels (RR: none, x only or general; the
+// sixth-order RR: x only or general; BGK: general), so that both give the same
+// result.
+template <int S, bool W, int OP>
+hipError_t launch_fused2_op(int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  const dim3 grid(8u * t.per_xcd), block(S == kF32 ? kT2PThreads : kT2Threads);
+  auto go = [&](auto fmc) {
+    constexpr int F = decltype(fmc)::value;
+    if constexpr (S == kF32)
+      hipLaunchKernelGGL((fused2p_kernel<W, F, OP>), grid, block, 0, s, a, t);
+    else
+      hipLaunchKernelGGL((fused2_kernel<S, W, F, OP>), grid, block, 0, s, a, t);
+  };
+  if constexpr (OP == kOpBGK) {
+    go(std::integral_constant<int, kForceAll>{});
+  } else if constexpr (OP == kOpRR6) {
+    if (fm == kForceX)
+      go(std::integral_constant<int, kForceX>{});
+    else
+      go(std::integral_constant<int, kForceAll>{});
+  } else {
+    switch (fm) {
+      case kForceNone: go(std::integral_constant<int, kForceNone>{}); break;
+      case kForceX: go(std::integral_constant<int, kForceX>{}); break;
+      default: go(std::integral_constant<int, kForceAll>{}); break;
+    }
   }
   return hipGetLastError();
 }
 
+template <int S, int OP>
+hipError_t launch_fused2_w(bool walls, int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  return walls ? launch_fused2_op<S, true, OP>(fm, a, t, s) : launch_fused2_op<S, false, OP>(fm, a, t, s);
+}
+
 template <int S>
-hipError_t launch_fused2_storage(bool walls, int fm, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+hipError_t launch_fused2_storage(bool walls, int op, int fm, const FusedArgs& a, const Fused2Args& t,
+                                 hipStream_t s) {
   if constexpr (S == kF16) {
     return hipErrorInvalidValue;
-  } else if constexpr (S == kF32) {
-    return walls ? launch_fused2p<true>(fm, a, t, s) : launch_fused2p<false>(fm, a, t, s);
-  } else {  // FP64
-    const dim3 grid(8u * t.per_xcd), block(kT2Threads);
-    if (walls) {
-      switch (fm) {
-        case kForceNone: hipLaunchKernelGGL((fused2_kernel<S, true, kForceNone>), grid, block, 0, s, a, t); break;
-        case kForceX: hipLaunchKernelGGL((fused2_kernel<S, true, kForceX>), grid, block, 0, s, a, t); break;
-        default: hipLaunchKernelGGL((fused2_kernel<S, true, kForceAll>), grid, block, 0, s, a, t); break;
-      }
-    } else {
-      switch (fm) {
-        case kForceNone: hipLaunchKernelGGL((fused2_kernel<S, false, kForceNone>), grid, block, 0, s, a, t); break;
-        case kForceX: hipLaunchKernelGGL((fused2_kernel<S, false, kForceX>), grid, block, 0, s, a, t); break;
-        default: hipLaunchKernelGGL((fused2_kernel<S, false, kForceAll>), grid, block, 0, s, a, t); break;
-      }
+  } else {
+    switch (op) {
+      case kOpRR: return launch_fused2_w<S, kOpRR>(walls, fm, a, t, s);
+      case kOpBGK: return launch_fused2_w<S, kOpBGK>(walls, fm, a, t, s);
+      case kOpRR6: return launch_fused2_w<S, kOpRR6>(walls, fm, a, t, s);
+      default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
   }
 }
 

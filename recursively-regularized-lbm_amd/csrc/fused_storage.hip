@@ -39,8 +39,8 @@ hipError_t launch_batched_s(Tag, const FusedArgs* table, const uint2* work, uint
   return launch_batched_s<kS>(table, work, ntiles, grid, parity, step, step_base, s);
 }
 
-hipError_t launch_fused2_s(Tag, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
-  return launch_fused2_storage<kS>(walls, force_mode_of(a), a, t, s);
+hipError_t launch_fused2_s(Tag, bool walls, int op, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+  return launch_fused2_storage<kS>(walls, op, force_mode_of(a), a, t, s);
 }
 
 hipError_t launch_fdbc_rows_s(Tag, const FusedArgs& a, hipStream_t s) {

@@ -168,7 +168,8 @@ struct StorageTag {};
                               uint32_t grid, uint32_t parity, uint32_t step, const uint32_t* step_base,             \
                               hipStream_t s);                                                                       \
   hipError_t launch_fdbc_rows_s(StorageTag<S>, const FusedArgs& a, hipStream_t s);                                 \
-  hipError_t launch_fused2_s(StorageTag<S>, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s);
+  hipError_t launch_fused2_s(StorageTag<S>, bool walls, int op, const FusedArgs& a, const Fused2Args& t,            \
+                             hipStream_t s);
 RRLBM_STORAGE_LAUNCHERS(kF64)
 RRLBM_STORAGE_LAUNCHERS(kF32)
 RRLBM_STORAGE_LAUNCHERS(kF16)
@@ -184,9 +185,10 @@ hipError_t launch_fused_op(int op, int storage, bool walls, int mode, const Fuse
                            hipStream_t s);
 // FDBC pre-pass: macroscopic velocity of rows 1, 2, LY-2, LY-3 into a.u_wall.
 hipError_t launch_fdbc_rows(int storage, const FusedArgs& a, hipStream_t s);
-// Two steps in one launch (D3Q27 RR without LES or per-cell force, FP32 / FP64
-// storage, lx a multiple of kT2X, ly of kT2Y): a.src -> a.dst.
-hipError_t launch_fused2(int storage, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s);
+// Two steps in one launch (D3Q27, op = kOpRR / kOpBGK / kOpRR6, without LES or
+// per-cell force, FP32 / FP64 storage, lx a multiple of kT2X, ly of kT2Y):
+// a.src -> a.dst.
+hipError_t launch_fused2(int storage, bool walls, int op, const FusedArgs& a, const Fused2Args& t, hipStream_t s);
 // Two cells per thread, STEP mode, D3Q27 without LES; a.nblocks = ceil(n / 512).
 hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32_t grid, hipStream_t s);
 // work[t] = (instance, block); grid workgroups (<= ntiles) stride over the list;

@@ -35,10 +35,10 @@ hipError_t launch_fused_pair(int storage, bool walls, const FusedArgs& a, uint32
   }
 }
 
-hipError_t launch_fused2(int storage, bool walls, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
+hipError_t launch_fused2(int storage, bool walls, int op, const FusedArgs& a, const Fused2Args& t, hipStream_t s) {
   switch (storage) {
-    case kF64: return launch_fused2_s(StorageTag<kF64>{}, walls, a, t, s);
-    case kF32: return launch_fused2_s(StorageTag<kF32>{}, walls, a, t, s);
+    case kF64: return launch_fused2_s(StorageTag<kF64>{}, walls, op, a, t, s);
+    case kF32: return launch_fused2_s(StorageTag<kF32>{}, walls, op, a, t, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -229,8 +229,8 @@ FusedArgs fused_args(rrlbm_ctx* c) {
 
 // Configurations the two-step kernel covers.
 bool two_step_ok(const rrlbm_params& p) {
-  return p.npop == 27 && !p.use_les && p.collision_model == RRLBM_COLLISION_RR && p.rr_order != 6 &&
-         p.wall_rr != 2 && p.storage != RRLBM_STORE_F16 && p.lx % (int)kT2X == 0 && p.ly % (int)kT2Y == 0;
+  return p.npop == 27 && !p.use_les && p.wall_rr != 2 && p.storage != RRLBM_STORE_F16 && p.lx % (int)kT2X == 0 &&
+         p.ly % (int)kT2Y == 0;
 }
 
 // Geometry of the two-step launch: tiles of kT2X x kT2Y cells, z chunks of
@@ -305,7 +305,8 @@ int fused_steps(rrlbm_ctx* c, int nsteps, const rrlbm_ctx::Forcing* next = nullp
       // two steps in one launch (the step it completes and both it collides
       // take the uniform force)
       const FusedArgs a = fused_args(c);
-      hipError_t e = launch_fused2(c->p.storage, has_walls(c->p), a, c->pass, c->stream);
+      const int op = is_bgk(c->p) ? kOpBGK : (is_rr6(c->p) ? kOpRR6 : kOpRR);
+      hipError_t e = launch_fused2(c->p.storage, has_walls(c->p), op, a, c->pass, c->stream);
       if (e != hipSuccess) return fail(c, RRLBM_E_HIP, "two-step kernel launch failed: %s", hipGetErrorString(e));
       c->sr_step += 2;
       std::swap(c->pop_a, c->pop_b);
@@ -529,7 +530,7 @@ int rrlbm_create(const rrlbm_params* pin, rrlbm_ctx** out) {
   if (p.steps_per_pass == 0) p.steps_per_pass = 1;
   if (p.steps_per_pass == 2 && !two_step_ok(p))
     return fail(nullptr, RRLBM_E_UNSUPPORTED,
-                "two steps per pass: the D3Q27 third-order RR step without LES or FDBC, FP32 or FP64 storage, "
+                "two steps per pass: the D3Q27 RR or BGK step without LES or FDBC, FP32 or FP64 storage, "
                 "lx a multiple of %u and ly of %u", kT2X, kT2Y);
   if (p.pass_planes < 0) return fail(nullptr, RRLBM_E_INVALID, "pass_planes must be >= 0");
   if (p.f16_rounding != RRLBM_F16_STOCHASTIC && p.f16_rounding != RRLBM_F16_NEAREST)

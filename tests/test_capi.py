@@ -74,13 +74,12 @@ def test_invalid_parameters_are_reported_not_fatal(rr):
              (dict(wall_rr=3), 1), (dict(wall_rr=2, wall_y=1, wall_x=1), 6), (dict(wall_rr=2, wall_y=1, ly=3), 6),
              (dict(wall_rr=2, wall_y=1, npop=19), 6), (dict(wall_rr=2, wall_y=1, rr_order=6), 6),
              (dict(wall_y=1, ly=1), 1), (dict(lid_face=6), 1),
-             # two steps per launch: the D3Q27 third-order RR step without LES or FDBC, FP32/FP64,
-             # lx a multiple of 64, ly of 4
+             # two steps per launch: the D3Q27 RR (orders 3, 6) and BGK steps without LES or FDBC,
+             # FP32/FP64, lx a multiple of 64, ly of 4
              (dict(steps_per_pass=3), 1), (dict(steps_per_pass=-1), 1), (dict(pass_planes=-1), 1),
              (dict(steps_per_pass=2, storage=2), 6), (dict(steps_per_pass=2, lx=96), 6),
              (dict(steps_per_pass=2, ly=66), 6), (dict(steps_per_pass=2, use_les=1), 6),
-             (dict(steps_per_pass=2, rr_order=6), 6), (dict(steps_per_pass=2, npop=19), 6),
-             (dict(steps_per_pass=2, collision_model=1, wall_rr=0), 6), (dict(steps_per_pass=2, wall_rr=2), 6)]
+             (dict(steps_per_pass=2, npop=19), 6), (dict(steps_per_pass=2, wall_rr=2), 6)]
     for bad, want in cases:
         p = rr.default_params(**bad)
         st = lib.rrlbm_create(ctypes.byref(p), ctypes.byref(h))

@@ -3,11 +3,13 @@
 The two-step kernel marches through z over tiles of 64 x 4 cells, computing
 the first step on the tile and a one-cell halo into a ring of planes in LDS
 and the second step from that ring.  It calls the one-step kernel's per-cell
-functions, so its states must equal those of one-step launches bit for bit
-(FP32 and FP64 storage, walls on any axis with the moving lid, uniform force
+functions, so its states must equal those of one-step launches: bit for bit
+where both kernels contract the same multiply-adds, else to an ulp (FP32 and
+FP64 storage, walls on any axis with the moving lid, uniform force
 along x or in any direction, odd step counts that end with a one-step launch,
-z chunks of 1, 2, 4 planes and the default, a grid of one plane).  The FP64
-run is also checked against the oracle's reference-order step (1e-12)."""
+z chunks of 1, 2, 4 planes and the default, a grid of one plane), for the RR
+collision and for the BGK and sixth-order RR operators.  The FP64 RR run is
+also checked against the oracle's reference-order step (1e-12)."""
 import numpy as np
 import pytest
 
@@ -48,6 +50,37 @@ def state(name):
 def test_two_step_equals_one_step_launches(rr, name, storage, planes):
     g, dims, params, force, walls, lid = state(name)
     steps = 8  # pack + 7 fused steps: three two-step launches and one one-step launch
+    f1, m1 = run(rr, g, steps, storage, force=force, **params)
+    f2, m2 = run(rr, g, steps, storage, force=force, steps_per_pass=2, pass_planes=planes, **params)
+    ef = maxabs(f1, f2)
+    em = max(maxabs(a, b) for a, b in zip(m1, m2))
+    print("%s storage %d planes %d: max |df| %.3g, max |d macro| %.3g, identical %s"
+          % (name, storage, planes, ef, em, np.array_equal(f1, f2)))
+    tol = 1e-15 if storage == 0 else 1e-6
+    assert ef < tol and em < tol, (ef, em)
+
+
+OP_CASES = {
+    # name: (dims, params, force); BGK walls are plain bounce-back (wall_rr 0)
+    "bgk_channel": ((64, 8, 6), dict(wall_y=1, wall_rr=0, collision_model=1), (3e-5, 0.0, 0.0)),
+    "bgk_periodic_force_xyz": ((128, 4, 5), dict(wall_rr=0, collision_model=1), (3e-5, 1e-6, -2e-6)),
+    "rr6_channel": ((64, 8, 6), dict(wall_y=1, wall_rr=1, rr_order=6), (3e-5, 0.0, 0.0)),
+    "rr6_channel_force_xyz": ((64, 8, 5), dict(wall_y=1, wall_rr=1, rr_order=6), (3e-5, 1e-6, -2e-6)),
+    "rr6_cavity": ((64, 8, 6), dict(wall_x=1, wall_y=1, wall_z=1, wall_rr=1, lid_face=3, lid_u=(0.05, 0.0, 0.0),
+                                    rr_order=6), (0.0, 0.0, 0.0)),
+}
+
+
+@pytest.mark.parametrize("planes", [0, 2])
+@pytest.mark.parametrize("storage", [0, 1])
+@pytest.mark.parametrize("name", sorted(OP_CASES))
+def test_two_step_operators_equal_one_step_launches(rr, name, storage, planes):
+    """The BGK operator and the sixth-order RR collision (with its wall
+    regularisation) on the two-step path equal one-step launches of the same
+    operator, whose kernels are checked against the drop-in reference order."""
+    dims, params, force = OP_CASES[name]
+    g = random_state(oracle.D3Q27_CUDA, *dims, seed=6, force=force)
+    steps = 8
     f1, m1 = run(rr, g, steps, storage, force=force, **params)
     f2, m2 = run(rr, g, steps, storage, force=force, steps_per_pass=2, pass_planes=planes, **params)
     ef = maxabs(f1, f2)
