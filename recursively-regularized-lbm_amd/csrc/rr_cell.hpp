@@ -263,134 +263,184 @@ RRLBM_HD RRLBM_INLINE Line3<C> line3_nok2(C k0, C k1) {
 // with scale_i = w_i when per_direction_w is true (giving g_i - w_i, the shifted
 // population) and scale_i = scale otherwise (FP16 storage: (g_i - w_i) S / w_i).
 // Collision: omega = 1 - 1/tau_eff, gs = +1.  RRBC: omega = 1, gs = -1.
+//
+// With per_direction_w the weights are folded into the polynomial instead of
+// 27 products at the end.  D3Q27's weights are products of the 1-D weights
+// (1, 4, 1)/6, so w_i = k(c_x) k(c_y) k(c_z) / 216 with k(0) = 4, k(+-1) = 1.
+// The coefficient of c_x^a c_y^b c_z^c is scaled by 4^(number of zero
+// exponents) / 216 -- the weight of a velocity with as many non-zero
+// components as (a, b, c) has non-zero exponents -- so that in each stage's
+// lines (k0, k1, k2) the constant term k0 carries a factor 4 against k1 and
+// k2, and the line's values at c = -1, 0, +1 are (k0/4 + k2 - k1, k0,
+// k0/4 + k2 + k1): each stage multiplies by k(c) at no cost, and the last one
+// leaves k(c_x) k(c_y) k(c_z) / 216 times the polynomial.  The powers of 4 go
+// into the constants (kF below; factors between terms are powers of 4, exact)
+// and the common 1/216 into rho, rho - 1 and omega, which scale every term:
+// all terms of a cell then carry the same rounding of that factor, and the
+// terms that cancel in the mass and momentum sums still cancel.
 template <typename C, bool per_direction_w, int FM = kForceAll>
 RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N, C Fx, C Fy, C Fz, C omega,
                                        C gs, C scale, C (&out)[27]) {
-  const C rho = q.rho, ux = q.ux, uy = q.uy, uz = q.uz;
+  // scale of a coefficient with n non-zero exponents, without the common
+  // 1/216 (all 1 without the weights)
+  constexpr double kF[4] = {per_direction_w ? 64.0 : 1.0, per_direction_w ? 16.0 : 1.0,
+                            per_direction_w ? 4.0 : 1.0, 1.0};
+  constexpr double kQ = per_direction_w ? 0.25 : 1.0;        // kF[n + 1] / kF[n]
+  constexpr double kS = per_direction_w ? 1.0 / 216.0 : 1.0;  // the common factor
+  const C ux = q.ux, uy = q.uy, uz = q.uz;
+  C rho = q.rho, dr = q.dr, wn = omega;
+  if constexpr (per_direction_w) {
+    rho *= C(kS);
+    dr *= C(kS);
+    wn *= C(kS);
+  }
   // feq / w as a polynomial in c: e_abc multiplies c_x^a c_y^b c_z^c
   const C ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
   const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
-  const C e000m1 = q.dr - C(1.5) * rho * (ux2 + uy2 + uz2);
-  const C e100 = rx * (C(3) - C(4.5) * (uy2 + uz2));
-  const C e010 = ry * (C(3) - C(4.5) * (ux2 + uz2));
-  const C e001 = rz * (C(3) - C(4.5) * (ux2 + uy2));
-  const C e200 = C(4.5) * rx * ux, e020 = C(4.5) * ry * uy, e002 = C(4.5) * rz * uz;
-  const C e110 = C(9) * rx * uy, e101 = C(9) * rx * uz, e011 = C(9) * ry * uz;
-  const C e210 = C(3) * e200 * uy, e201 = C(3) * e200 * uz;
-  const C e120 = C(3) * e020 * ux, e021 = C(3) * e020 * uz;
-  const C e102 = C(3) * e002 * ux, e012 = C(3) * e002 * uy;
-  const C e111 = C(3) * e110 * uz;
+  const C e000m1 = C(kF[0]) * dr - C(1.5 * kF[0]) * rho * (ux2 + uy2 + uz2);
+  const C e100 = rx * (C(3 * kF[1]) - C(4.5 * kF[1]) * (uy2 + uz2));
+  const C e010 = ry * (C(3 * kF[1]) - C(4.5 * kF[1]) * (ux2 + uz2));
+  const C e001 = rz * (C(3 * kF[1]) - C(4.5 * kF[1]) * (ux2 + uy2));
+  const C e200 = C(4.5 * kF[1]) * rx * ux, e020 = C(4.5 * kF[1]) * ry * uy, e002 = C(4.5 * kF[1]) * rz * uz;
+  const C e110 = C(9 * kF[2]) * rx * uy, e101 = C(9 * kF[2]) * rx * uz, e011 = C(9 * kF[2]) * ry * uz;
+  const C e210 = C(3 * kQ) * e200 * uy, e201 = C(3 * kQ) * e200 * uz;
+  const C e120 = C(3 * kQ) * e020 * ux, e021 = C(3 * kQ) * e020 * uz;
+  const C e102 = C(3 * kQ) * e002 * ux, e012 = C(3 * kQ) * e002 * uy;
+  const C e111 = C(3 * kQ) * e110 * uz;
   // fneq / w: same structure with N and the recursive third-order tensor R
   const C Rxy = N.xx * uy + C(2) * ux * N.xy, Rxz = N.xx * uz + C(2) * ux * N.xz;
   const C Ryx = N.yy * ux + C(2) * uy * N.xy, Ryz = N.yy * uz + C(2) * uy * N.yz;
   const C Rzx = N.zz * ux + C(2) * uz * N.xz, Rzy = N.zz * uy + C(2) * uz * N.yz;
   const C Rxyz = N.xy * uz + N.xz * uy + N.yz * ux;
-  const C wn = omega;
-  const C n000 = C(-1.5) * (N.xx + N.yy + N.zz);
   // Coefficients K of feq (1 + gs Fbar) + omega fneq, Fbar_i = 3/2 (c_i - u).F:
   // K = b0 e + [phi.c e reduced with c^3 = c] + omega n, b0 = 1 - 3/2 gs u.F,
   // phi = 3/2 gs F.  Products with absent force components are not formed.
-  C K000 = e000m1 + wn * n000;
-  C K100 = e100 + wn * (C(-4.5) * (Ryx + Rzx));
-  C K010 = e010 + wn * (C(-4.5) * (Rxy + Rzy));
-  C K001 = e001 + wn * (C(-4.5) * (Rxz + Ryz));
-  C K200 = e200 + wn * (C(4.5) * N.xx);
-  C K020 = e020 + wn * (C(4.5) * N.yy);
-  C K002 = e002 + wn * (C(4.5) * N.zz);
-  C K110 = e110 + wn * (C(9) * N.xy);
-  C K101 = e101 + wn * (C(9) * N.xz);
-  C K011 = e011 + wn * (C(9) * N.yz);
-  C K210 = e210 + wn * (C(13.5) * Rxy);
-  C K201 = e201 + wn * (C(13.5) * Rxz);
-  C K120 = e120 + wn * (C(13.5) * Ryx);
-  C K021 = e021 + wn * (C(13.5) * Ryz);
-  C K102 = e102 + wn * (C(13.5) * Rzx);
-  C K012 = e012 + wn * (C(13.5) * Rzy);
-  C K111 = e111 + wn * (C(27) * Rxyz);
+  // omega k x: with per_direction_w as (omega k) x -- omega is uniform in the
+  // kernels with a constant relaxation time, so omega k is formed once per
+  // launch where a workgroup loops over cells; the FP16 form keeps omega (k x)
+  auto wt = [&](double k, C x) -> C {
+    if constexpr (per_direction_w)
+      return (wn * C(k)) * x;
+    else
+      return wn * (C(k) * x);
+  };
+  C K000 = e000m1 + wt(-1.5 * kF[0], N.xx + N.yy + N.zz);
+  C K100 = e100 + wt(-4.5 * kF[1], Ryx + Rzx);
+  C K010 = e010 + wt(-4.5 * kF[1], Rxy + Rzy);
+  C K001 = e001 + wt(-4.5 * kF[1], Rxz + Ryz);
+  C K200 = e200 + wt(4.5 * kF[1], N.xx);
+  C K020 = e020 + wt(4.5 * kF[1], N.yy);
+  C K002 = e002 + wt(4.5 * kF[1], N.zz);
+  C K110 = e110 + wt(9 * kF[2], N.xy);
+  C K101 = e101 + wt(9 * kF[2], N.xz);
+  C K011 = e011 + wt(9 * kF[2], N.yz);
+  C K210 = e210 + wt(13.5 * kF[2], Rxy);
+  C K201 = e201 + wt(13.5 * kF[2], Rxz);
+  C K120 = e120 + wt(13.5 * kF[2], Ryx);
+  C K021 = e021 + wt(13.5 * kF[2], Ryz);
+  C K102 = e102 + wt(13.5 * kF[2], Rzx);
+  C K012 = e012 + wt(13.5 * kF[2], Rzy);
+  C K111 = e111 + wt(27 * kF[3], Rxyz);
   C K220, K202, K022, K211, K121, K112;
   if constexpr (FM == kForceNone) {
     K220 = K202 = K022 = K211 = K121 = K112 = C(0);
   } else {
     const C k15 = C(1.5) * gs;
     const C phx = k15 * Fx;
-    const C e000 = C(1) + e000m1;
+    const C e000 = C(kF[0] * kS) + e000m1;
     C bm1 = -(phx * ux);
     if constexpr (FM == kForceAll) bm1 -= k15 * (Fy * uy + Fz * uz);
     // b0 e - e = bm1 e
-    K000 += bm1 + bm1 * e000m1;
-    K100 += bm1 * e100 + phx * (e000 + e200);
+    K000 += bm1 * C(kF[0] * kS) + bm1 * e000m1;
+    K100 += bm1 * e100 + phx * (C(kQ) * e000 + e200);
     K010 += bm1 * e010;
     K001 += bm1 * e001;
     K200 += bm1 * e200 + phx * e100;
     K020 += bm1 * e020;
     K002 += bm1 * e002;
-    K110 += bm1 * e110 + phx * (e010 + e210);
-    K101 += bm1 * e101 + phx * (e001 + e201);
+    K110 += bm1 * e110 + phx * (C(kQ) * e010 + e210);
+    K101 += bm1 * e101 + phx * (C(kQ) * e001 + e201);
     K011 += bm1 * e011;
     K210 += bm1 * e210 + phx * e110;
     K201 += bm1 * e201 + phx * e101;
-    K120 += bm1 * e120 + phx * e020;
+    K120 += bm1 * e120 + phx * C(kQ) * e020;
     K021 += bm1 * e021;
-    K102 += bm1 * e102 + phx * e002;
+    K102 += bm1 * e102 + phx * C(kQ) * e002;
     K012 += bm1 * e012;
-    K111 += bm1 * e111 + phx * e011;
+    K111 += bm1 * e111 + phx * C(kQ) * e011;
     K220 = phx * e120;
     K202 = phx * e102;
     K022 = C(0);
     K211 = phx * e111;
-    K121 = phx * e021;
-    K112 = phx * e012;
+    K121 = phx * C(kQ) * e021;
+    K112 = phx * C(kQ) * e012;
     if constexpr (FM == kForceAll) {
       const C phy = k15 * Fy, phz = k15 * Fz;
-      K010 += phy * (e000 + e020);
-      K001 += phz * (e000 + e002);
+      K010 += phy * (C(kQ) * e000 + e020);
+      K001 += phz * (C(kQ) * e000 + e002);
       K020 += phy * e010;
       K002 += phz * e001;
-      K110 += phy * (e100 + e120);
-      K101 += phz * (e100 + e102);
-      K011 += phy * (e001 + e021) + phz * (e010 + e012);
-      K210 += phy * e200;
-      K201 += phz * e200;
+      K110 += phy * (C(kQ) * e100 + e120);
+      K101 += phz * (C(kQ) * e100 + e102);
+      K011 += phy * (C(kQ) * e001 + e021) + phz * (C(kQ) * e010 + e012);
+      K210 += phy * C(kQ) * e200;
+      K201 += phz * C(kQ) * e200;
       K120 += phy * e110;
-      K021 += phy * e011 + phz * e020;
+      K021 += phy * e011 + phz * C(kQ) * e020;
       K102 += phz * e101;
-      K012 += phy * e002 + phz * e011;
-      K111 += phy * e101 + phz * e110;
+      K012 += phy * C(kQ) * e002 + phz * e011;
+      K111 += phy * C(kQ) * e101 + phz * C(kQ) * e110;
       K220 += phy * e210;
       K202 += phz * e201;
       K022 = phy * e012 + phz * e021;
-      K211 += phy * e201 + phz * e210;
-      K121 += phy * e111 + phz * e120;
-      K112 += phy * e102 + phz * e111;
+      K211 += phy * C(kQ) * e201 + phz * C(kQ) * e210;
+      K121 += phy * e111 + phz * C(kQ) * e120;
+      K112 += phy * C(kQ) * e102 + phz * e111;
     }
   }
   // tensor-product evaluation: x, then y, then z
   constexpr bool kq = FM != kForceNone;   // K220, K202, K211, K121, K112 present
   constexpr bool k22 = FM == kForceAll;   // K022 present
-  const Line3<C> L00 = line3(K000, K100, K200);
-  const Line3<C> L10 = line3(K010, K110, K210);
-  const Line3<C> L01 = line3(K001, K101, K201);
-  const Line3<C> L20 = kq ? line3(K020, K120, K220) : line3_nok2(K020, K120);
-  const Line3<C> L02 = kq ? line3(K002, K102, K202) : line3_nok2(K002, K102);
-  const Line3<C> L11 = kq ? line3(K011, K111, K211) : line3_nok2(K011, K111);
-  const Line3<C> L21 = line3_nok2(K021, K121);
-  const Line3<C> L12 = line3_nok2(K012, K112);
-  const C L22 = K022;
+  auto ln = [](C k0, C k1, C k2) -> Line3<C> {
+    if constexpr (per_direction_w) {
+      const C t = fma(k0, C(0.25), k2);
+      return {t - k1, k0, t + k1};
+    } else {
+      return line3(k0, k1, k2);
+    }
+  };
+  auto ln_nok2 = [](C k0, C k1) -> Line3<C> {
+    if constexpr (per_direction_w)
+      return {C(fma(k0, C(0.25), -k1)), k0, C(fma(k0, C(0.25), k1))};
+    else
+      return line3_nok2(k0, k1);
+  };
+  const Line3<C> L00 = ln(K000, K100, K200);
+  const Line3<C> L10 = ln(K010, K110, K210);
+  const Line3<C> L01 = ln(K001, K101, K201);
+  const Line3<C> L20 = kq ? ln(K020, K120, K220) : ln_nok2(K020, K120);
+  const Line3<C> L02 = kq ? ln(K002, K102, K202) : ln_nok2(K002, K102);
+  const Line3<C> L11 = kq ? ln(K011, K111, K211) : ln_nok2(K011, K111);
+  const Line3<C> L21 = ln_nok2(K021, K121);
+  const Line3<C> L12 = ln_nok2(K012, K112);
+  // the line of K022 alone: K022 at c_x = 0, K022 / 4 at c_x = +-1 (weighted)
+  const C L22 = K022, L22q = C(kQ) * K022;
   auto pick = [](const Line3<C>& l, int c) -> C { return c < 0 ? l.m : (c == 0 ? l.z : l.p); };
   static_for<3>([&](auto jx) {
     constexpr int cx = decltype(jx)::value - 1;
-    const Line3<C> M0 = line3(pick(L00, cx), pick(L10, cx), pick(L20, cx));
-    const Line3<C> M1 = line3(pick(L01, cx), pick(L11, cx), pick(L21, cx));
-    const Line3<C> M2 = k22 ? line3(pick(L02, cx), pick(L12, cx), L22) : line3_nok2(pick(L02, cx), pick(L12, cx));
+    const Line3<C> M0 = ln(pick(L00, cx), pick(L10, cx), pick(L20, cx));
+    const Line3<C> M1 = ln(pick(L01, cx), pick(L11, cx), pick(L21, cx));
+    const Line3<C> M2 = k22 ? ln(pick(L02, cx), pick(L12, cx), cx == 0 ? L22 : L22q)
+                            : ln_nok2(pick(L02, cx), pick(L12, cx));
     static_for<3>([&](auto jy) {
       constexpr int cy = decltype(jy)::value - 1;
-      const Line3<C> V = line3(pick(M0, cy), pick(M1, cy), pick(M2, cy));
+      const Line3<C> V = ln(pick(M0, cy), pick(M1, cy), pick(M2, cy));
       static_for<3>([&](auto jz) {
         constexpr int cz = decltype(jz)::value - 1;
         constexpr int i = D3Q27::index(cx, cy, cz);
         const C val = pick(V, cz);
         if constexpr (per_direction_w)
-          out[i] = val * C(D3Q27::w(i));
+          out[i] = val;
         else
           out[i] = val * scale;
       });
