@@ -277,7 +277,8 @@ RRLBM_HD RRLBM_INLINE Line3<C> line3_nok2(C k0, C k1) {
 // into the constants (kF below; factors between terms are powers of 4, exact)
 // and the common 1/216 into rho, rho - 1 and omega, which scale every term:
 // all terms of a cell then carry the same rounding of that factor, and the
-// terms that cancel in the mass and momentum sums still cancel.
+// terms that cancel in the mass and momentum sums still cancel.  Without the
+// weights the output scale goes in the same way.
 template <typename C, bool per_direction_w, int FM = kForceAll>
 RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N, C Fx, C Fy, C Fz, C omega,
                                        C gs, C scale, C (&out)[27]) {
@@ -285,15 +286,15 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
   // 1/216 (all 1 without the weights)
   constexpr double kF[4] = {per_direction_w ? 64.0 : 1.0, per_direction_w ? 16.0 : 1.0,
                             per_direction_w ? 4.0 : 1.0, 1.0};
-  constexpr double kQ = per_direction_w ? 0.25 : 1.0;        // kF[n + 1] / kF[n]
-  constexpr double kS = per_direction_w ? 1.0 / 216.0 : 1.0;  // the common factor
+  constexpr double kQ = per_direction_w ? 0.25 : 1.0;  // kF[n + 1] / kF[n]
+  // the common factor: 1/216 with the weights, the output scale without
+  C sc;
+  if constexpr (per_direction_w)
+    sc = C(1.0 / 216.0);
+  else
+    sc = scale;
   const C ux = q.ux, uy = q.uy, uz = q.uz;
-  C rho = q.rho, dr = q.dr, wn = omega;
-  if constexpr (per_direction_w) {
-    rho *= C(kS);
-    dr *= C(kS);
-    wn *= C(kS);
-  }
+  const C rho = q.rho * sc, dr = q.dr * sc, wn = omega * sc;
   // feq / w as a polynomial in c: e_abc multiplies c_x^a c_y^b c_z^c
   const C ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
   const C rx = rho * ux, ry = rho * uy, rz = rho * uz;
@@ -315,15 +316,10 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
   // Coefficients K of feq (1 + gs Fbar) + omega fneq, Fbar_i = 3/2 (c_i - u).F:
   // K = b0 e + [phi.c e reduced with c^3 = c] + omega n, b0 = 1 - 3/2 gs u.F,
   // phi = 3/2 gs F.  Products with absent force components are not formed.
-  // omega k x: with per_direction_w as (omega k) x -- omega is uniform in the
-  // kernels with a constant relaxation time, so omega k is formed once per
-  // launch where a workgroup loops over cells; the FP16 form keeps omega (k x)
-  auto wt = [&](double k, C x) -> C {
-    if constexpr (per_direction_w)
-      return (wn * C(k)) * x;
-    else
-      return wn * (C(k) * x);
-  };
+  // omega k x as (omega k) x: omega is uniform in the kernels with a constant
+  // relaxation time, so omega k is formed once per launch where a workgroup
+  // loops over cells
+  auto wt = [&](double k, C x) -> C { return (wn * C(k)) * x; };
   C K000 = e000m1 + wt(-1.5 * kF[0], N.xx + N.yy + N.zz);
   C K100 = e100 + wt(-4.5 * kF[1], Ryx + Rzx);
   C K010 = e010 + wt(-4.5 * kF[1], Rxy + Rzy);
@@ -347,11 +343,11 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
   } else {
     const C k15 = C(1.5) * gs;
     const C phx = k15 * Fx;
-    const C e000 = C(kF[0] * kS) + e000m1;
+    const C e000 = C(kF[0]) * sc + e000m1;
     C bm1 = -(phx * ux);
     if constexpr (FM == kForceAll) bm1 -= k15 * (Fy * uy + Fz * uz);
     // b0 e - e = bm1 e
-    K000 += bm1 * C(kF[0] * kS) + bm1 * e000m1;
+    K000 += bm1 * (C(kF[0]) * sc) + bm1 * e000m1;
     K100 += bm1 * e100 + phx * (C(kQ) * e000 + e200);
     K010 += bm1 * e010;
     K001 += bm1 * e001;
@@ -438,11 +434,7 @@ RRLBM_HD RRLBM_INLINE void rr_post_q27(const Macro<C>& q, const NeqTensor<C>& N,
       static_for<3>([&](auto jz) {
         constexpr int cz = decltype(jz)::value - 1;
         constexpr int i = D3Q27::index(cx, cy, cz);
-        const C val = pick(V, cz);
-        if constexpr (per_direction_w)
-          out[i] = val;
-        else
-          out[i] = val * scale;
+        out[i] = pick(V, cz);
       });
     });
   });

@@ -67,8 +67,8 @@ def test_taylor_green_c1_energy_decay(rr):
     Measured on MI355X after 2000 steps: FP32 +9.5e-5 relative (the same on
     every run: FP32 rounding of the stored populations against a viscous
     non-equilibrium part 5e-4 times smaller); FP16 with stochastic rounding
-    +1.07e-4 +- 2.2e-5 over eight dither realisations (u0 changed by 1e-6
-    relative; 8.1e-5 on this one), with round-to-nearest +4.3e-3 to +4.6e-3
+    +1.0e-4 +- 4.5e-5 over eight dither realisations (u0 changed by 1e-6
+    relative; at most 1.63e-4), with round-to-nearest +4.3e-3 to +4.6e-3
     (the decay per step is below half an FP16 step of the stored values)."""
     steps = [500, 1000, 1500, 2000]
     ke64, nu = tg2d_run(rr, 0, steps)
