@@ -7,7 +7,8 @@ functions, so its states must equal those of one-step launches: bit for bit
 where both kernels contract the same multiply-adds, else to an ulp (FP32 and
 FP64 storage, walls on any axis with the moving lid, uniform force
 along x or in any direction, odd step counts that end with a one-step launch,
-z chunks of 1, 2, 4 planes and the default, a grid of one plane), for the RR
+z chunks of 1, 2, 4, 64 planes and the default, grids of one and two planes,
+several tiles in x and y, one tile in y), for the RR
 collision and for the BGK and sixth-order RR operators.  The FP64 RR run is
 also checked against the oracle's reference-order step (1e-12)."""
 import numpy as np
@@ -27,6 +28,13 @@ CASES = {
     "periodic_one_plane": ((64, 4, 1), dict(wall_y=0, wall_rr=0), (2e-5, 0.0, 0.0), (0, 0, 0), -1),
     "cavity": ((64, 8, 6), dict(wall_x=1, wall_y=1, wall_z=1, wall_rr=1, lid_face=3, lid_u=(0.05, 0.0, 0.0)),
                (0.0, 0.0, 0.0), (1, 1, 1), 3),
+    # three tiles in x and y (nine work items, fewer than the eight XCDs' runs
+    # fill evenly), walls on x and y
+    "duct_3x3_tiles": ((192, 12, 5), dict(wall_x=1, wall_y=1, wall_rr=1), (3e-5, 0.0, 0.0), (1, 1, 0), -1),
+    # one tile in y: both halo rows wrap onto the wall rows
+    "channel_one_tile_row": ((64, 4, 5), dict(wall_y=1, wall_rr=1), (3e-5, 0.0, 0.0), (0, 1, 0), -1),
+    # two planes between z walls: every chunk border is a wall plane
+    "two_planes_z_walls": ((64, 8, 2), dict(wall_y=0, wall_z=1, wall_rr=1), (0.0, 0.0, 0.0), (0, 0, 1), -1),
 }
 
 
@@ -44,7 +52,7 @@ def state(name):
     return random_state(oracle.D3Q27_CUDA, *dims, seed=5, force=force), dims, params, force, walls, lid
 
 
-@pytest.mark.parametrize("planes", [0, 1, 2, 4])
+@pytest.mark.parametrize("planes", [0, 1, 2, 4, 64])
 @pytest.mark.parametrize("storage", [0, 1])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_two_step_equals_one_step_launches(rr, name, storage, planes):
@@ -91,7 +99,7 @@ def test_two_step_operators_equal_one_step_launches(rr, name, storage, planes):
     assert ef < tol and em < tol, (ef, em)
 
 
-@pytest.mark.parametrize("name", ["channel", "cavity", "periodic"])
+@pytest.mark.parametrize("name", ["channel", "cavity", "periodic", "duct_3x3_tiles", "two_planes_z_walls"])
 def test_two_step_fp64_matches_oracle(rr, name):
     g, dims, params, force, walls, lid = state(name)
     steps = 9
