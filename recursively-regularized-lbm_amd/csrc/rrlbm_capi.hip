@@ -1295,10 +1295,16 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
         if (cpt == 2 && !pair_ok) continue;
         shapes.push_back({base.group_rows, base.pad, cpt, remap, pers});
       }
-  // two steps per launch, z chunks of 8 to 32 planes
-  if (two_step_ok(c->p))
-    for (int planes : {8, 12, 16, 24, 32})
-      if (planes <= c->p.lz) shapes.push_back({base.group_rows, base.pad, 1, base.remap, 0, 2, planes});
+  // two steps per launch, z chunks of 8 to 32 planes, with the layout the
+  // one-step trials chose and with the library default layout: the two kernels
+  // do not always prefer the same one (structure of arrays can win the one-step
+  // trials and run the two-step kernel 8 % slower than 16-row groups)
+  if (two_step_ok(c->p)) {
+    const Candidate lay_default{0, -1, 1, 0, 0};
+    for (const Candidate& l : {base, lay_default})
+      for (int planes : {8, 12, 16, 24, 32})
+        if (planes <= c->p.lz) shapes.push_back({l.group_rows, l.pad, 1, base.remap, 0, 2, planes});
+  }
   std::vector<double> ms2;
   int b2 = 0;
   rc = time_candidates(c, shapes, trial_steps, &ms2, &b2);
@@ -1324,8 +1330,8 @@ int rrlbm_autotune(rrlbm_ctx* c, int trial_steps, rrlbm_tuning* out) {
     if (rc) return rc;
   }
   c->sr_step = sr0;
-  out->pop_group_rows = best.group_rows;
-  out->pop_pad_bytes = best.pad;
+  out->pop_group_rows = c->p.pop_group_rows;  // resolved (a candidate may hold the defaults 0 / -1)
+  out->pop_pad_bytes = c->p.pop_pad_bytes;
   out->cells_per_thread = best.cpt;
   out->xcd_remap = best.remap;
   out->persistent_wg_per_cu = best.pers;
