@@ -1091,8 +1091,13 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
   // The halo-column wavefront gathers with per-lane addresses when its plane
   // starts (it is not prefetched: holding its next gather raised the register
   // count and cost 9 %), so it finishes its plane last; issue priority over the
-  // other wavefronts of its SIMD shortens each plane by 6 %.
-  if (w == kH2Y) __builtin_amdgcn_s_setprio(3);
+  // other wavefronts of its SIMD shortens each plane by 6 %, and the step-t+2
+  // wavefronts ahead of the halo rows (whose gathers were issued a plane
+  // earlier) by another 2 %.
+  if (w == kH2Y)
+    __builtin_amdgcn_s_setprio(3);
+  else if (w > kH2Y)
+    __builtin_amdgcn_s_setprio(1);
   // step t+1 cell (wavefronts 0-5: halo row w, columns 1..64; wavefront 6
   // lanes 0-11: the halo columns 0 and 65)
   uint32_t hx, hy;
