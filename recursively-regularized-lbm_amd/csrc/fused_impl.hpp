@@ -1089,8 +1089,8 @@ __global__ __launch_bounds__(kT2PThreads) __attribute__((amdgpu_waves_per_eu(3))
   const uint32_t tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
   const bool step1 = w <= kH2Y;  // wavefronts 0-6
   // The halo-column wavefront gathers with per-lane addresses when its plane
-  // starts (it is not prefetched: holding its next gather raised the register
-  // count and cost 9 %), so it finishes its plane last; issue priority over the
+  // starts (prefetched like the rows' gathers it measured 10 % slower, with no
+  // register spills), so it finishes its plane last; issue priority over the
   // other wavefronts of its SIMD shortens each plane by 6 %, and the step-t+2
   // wavefronts ahead of the halo rows (whose gathers were issued a plane
   // earlier) by another 2 %.
